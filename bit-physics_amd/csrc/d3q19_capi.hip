@@ -1,0 +1,681 @@
+// d3q19_capi.hip -- C ABI of the D3Q19 engine (declared in include/d3q19.h).
+//
+// A lattice owns two population planes sets (ping-pong) holding the
+// post-collision deviations of the current step; the reference's
+// pre-collision state f^n is produced on demand by the read-back kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/d3q19.h"
+#include "d3q19_kernels.hpp"
+
+using namespace d3q19;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(D3Q19_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + BLOCK - 1) / BLOCK); }
+
+size_t storage_bytes(int s) {
+    switch (s) {
+        case D3Q19_F64: return 8;
+        case D3Q19_F32: return 4;
+        case D3Q19_F16:
+        case D3Q19_BF16: return 2;
+    }
+    return 0;
+}
+
+// Device scratch allocation that frees itself.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipMalloc(&p, n); }
+};
+
+}  // namespace
+
+struct d3q19_lattice {
+    d3q19_config cfg;
+    hipStream_t stream = nullptr;
+    hipStream_t own_stream = nullptr;
+    uint32_t ncell = 0;
+    size_t stride = 0;  // elements between direction planes
+    size_t esize = 0;
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    void* force = nullptr;   // compute-type (3, ncell)
+    void* brink = nullptr;   // compute-type (ncell)
+    uint8_t* solid = nullptr;
+    double* partials = nullptr;
+    double* dout = nullptr;  // device results of reductions
+    float* uscratch = nullptr;
+    bool has_state = false;
+    int64_t steps = 0;
+    size_t bytes = 0;
+};
+
+namespace {
+
+template <class C>
+Params<C> make_params(const d3q19_lattice* L) {
+    Params<C> p;
+    std::memset(&p, 0, sizeof(p));
+    p.nx = (uint32_t)L->cfg.nx;
+    p.ny = (uint32_t)L->cfg.ny;
+    p.nz = (uint32_t)L->cfg.nz;
+    p.ncell = L->ncell;
+    p.omega = (C)(1.0 / L->cfg.tau);
+    p.pref = (C)(1.0 - 0.5 / L->cfg.tau);
+    p.force_mode = L->cfg.force_mode;
+    p.wall_mask = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->cfg.wall_faces : 0u;
+    for (int a = 0; a < 3; ++a) {
+        p.fu[a] = (C)L->cfg.force[a];
+        p.wtop[a] = (C)L->cfg.wall_velocity_top[a];
+        p.wbot[a] = (C)L->cfg.wall_velocity_bottom[a];
+        for (int k = 0; k < 6; ++k) p.face_u[k][a] = (C)L->cfg.face_velocity[k][a];
+    }
+    p.force = (const C*)L->force;
+    p.brink = (const C*)L->brink;
+    p.solid = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->solid : nullptr;
+    return p;
+}
+
+template <class Tag>
+Planes<Tag> planes(const d3q19_lattice* L, int which) {
+    Planes<Tag> pl;
+    typedef typename Codec<Tag>::raw R;
+    R* base = (R*)L->buf[which];
+    for (int i = 0; i < Q; ++i) pl.p[i] = base + (size_t)i * L->stride;
+    return pl;
+}
+
+template <class Tag>
+CPlanes<Tag> cplanes(const d3q19_lattice* L, int which) {
+    CPlanes<Tag> pl;
+    Planes<Tag> q = planes<Tag>(L, which);
+    for (int i = 0; i < Q; ++i) pl.p[i] = q.p[i];
+    return pl;
+}
+
+template <class Tag>
+int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const uint32_t g = nblocks(L->ncell);
+    for (int32_t s = 0; s < n; ++s) {
+        CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
+        Planes<Tag> out = planes<Tag>(L, L->cur ^ 1);
+        float* r = (s == n - 1) ? rho : nullptr;
+        float* v = (s == n - 1) ? u : nullptr;
+        switch (L->cfg.boundary) {
+            case D3Q19_BC_PERIODIC:
+                k_step<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                break;
+            case D3Q19_BC_YWALLS:
+                k_step<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                break;
+            default:
+                k_step<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                break;
+        }
+        HIPCHK(hipGetLastError());
+        L->cur ^= 1;
+        L->steps++;
+    }
+    return D3Q19_OK;
+}
+
+template <class Tag, class In>
+int launch_init(d3q19_lattice* L, const In* f_dev) {
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    k_init<Tag, In><<<nblocks(L->ncell), BLOCK, 0, L->stream>>>(f_dev, planes<Tag>(L, L->cur), p);
+    HIPCHK(hipGetLastError());
+    L->has_state = true;
+    L->steps = 0;
+    return D3Q19_OK;
+}
+
+template <class Tag, class Out>
+int launch_readback(d3q19_lattice* L, Out* f, Out* rho, Out* u) {
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const uint32_t g = nblocks(L->ncell);
+    CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_readback<Tag, BC_PERIODIC, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_readback<Tag, BC_YWALLS, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+        default:
+            k_readback<Tag, BC_HALFWAY, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+template <class Tag>
+int launch_diag(d3q19_lattice* L) {
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const uint32_t g = nblocks(L->ncell);
+    CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_diag<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_diag<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            break;
+        default:
+            k_diag<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            break;
+    }
+    HIPCHK(hipGetLastError());
+    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, g, NDIAG, L->dout);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+// Dispatch on the lattice storage type.
+#define D3Q19_DISPATCH(L, FN, ...)                                              \
+    [&]() -> int {                                                              \
+        switch ((L)->cfg.storage) {                                             \
+            case D3Q19_F64: return FN<TF64>(__VA_ARGS__);                       \
+            case D3Q19_F32: return FN<TF32>(__VA_ARGS__);                       \
+            case D3Q19_F16: return FN<TF16>(__VA_ARGS__);                       \
+            case D3Q19_BF16: return FN<TBF16>(__VA_ARGS__);                     \
+        }                                                                       \
+        return fail(D3Q19_EINVAL, "bad storage type");                          \
+    }()
+
+bool is_ct_double(const d3q19_lattice* L) { return L->cfg.storage == D3Q19_F64; }
+size_t ct_size(const d3q19_lattice* L) { return is_ct_double(L) ? 8 : 4; }
+
+// Stage a host or device array of n elements of dtype `dt` into a device
+// buffer of the lattice's compute type (double or float).
+__global__ void k_convert_f32_to_f64(const float* a, double* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+__global__ void k_convert_f64_to_f32(const double* a, float* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+int stage_to_compute(d3q19_lattice* L, const void* src, int32_t dt, int32_t on_device, size_t n,
+                     void* dst) {
+    if (dt != D3Q19_F64 && dt != D3Q19_F32) return fail(D3Q19_EINVAL, "dtype must be F64 or F32");
+    const size_t in_es = dt == D3Q19_F64 ? 8 : 4;
+    const bool want_double = is_ct_double(L);
+    const bool same = (dt == D3Q19_F64) == want_double;
+    if (same) {
+        HIPCHK(hipMemcpyAsync(dst, src, n * in_es,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->stream));
+        if (!on_device) HIPCHK(hipStreamSynchronize(L->stream));
+        return D3Q19_OK;
+    }
+    DevBuf tmp;
+    const void* dsrc = src;
+    if (!on_device) {
+        if (tmp.alloc(n * in_es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+        HIPCHK(hipMemcpyAsync(tmp.p, src, n * in_es, hipMemcpyHostToDevice, L->stream));
+        dsrc = tmp.p;
+    }
+    if (want_double)
+        k_convert_f32_to_f64<<<nblocks(n), BLOCK, 0, L->stream>>>((const float*)dsrc, (double*)dst, n);
+    else
+        k_convert_f64_to_f32<<<nblocks(n), BLOCK, 0, L->stream>>>((const double*)dsrc, (float*)dst, n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int check_lat(const d3q19_lattice* L) {
+    if (!L) return fail(D3Q19_EINVAL, "null lattice handle");
+    return D3Q19_OK;
+}
+
+int check_dtype(int32_t dt) {
+    if (dt != D3Q19_F64 && dt != D3Q19_F32) return fail(D3Q19_EINVAL, "dtype must be D3Q19_F64 or D3Q19_F32");
+    return D3Q19_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* d3q19_last_error(void) { return g_err.c_str(); }
+const char* d3q19_version(void) { return "d3q19-mi355x 0.1.0 (gfx950)"; }
+
+void d3q19_config_default(d3q19_config* cfg) {
+    if (!cfg) return;
+    std::memset(cfg, 0, sizeof(*cfg));
+    cfg->tau = 0.6;
+    cfg->storage = D3Q19_F32;
+    cfg->boundary = D3Q19_BC_PERIODIC;
+    cfg->force_mode = D3Q19_FORCE_NONE;
+}
+
+int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
+    if (!cfg || !out) return fail(D3Q19_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
+    const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
+    const size_t es = storage_bytes(cfg->storage);
+    if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16 or BF16");
+    if (ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (ncell * 8 must fit 32 bits)");
+    if (!(cfg->tau > 0.5)) return fail(D3Q19_EINVAL, "tau must be > 0.5");
+    if (cfg->boundary < D3Q19_BC_PERIODIC || cfg->boundary > D3Q19_BC_HALFWAY)
+        return fail(D3Q19_EINVAL, "bad boundary mode");
+    if (cfg->force_mode < D3Q19_FORCE_NONE || cfg->force_mode > D3Q19_FORCE_FIELD)
+        return fail(D3Q19_EINVAL, "bad force mode");
+    if (cfg->boundary == D3Q19_BC_HALFWAY && (cfg->wall_faces & ~63u))
+        return fail(D3Q19_EINVAL, "wall_faces has bits beyond the six faces");
+    if (hipSetDevice(cfg->device) != hipSuccess) return fail(D3Q19_EHIP, "hipSetDevice failed");
+
+    d3q19_lattice* L = new d3q19_lattice();
+    L->cfg = *cfg;
+    L->ncell = (uint32_t)ncell;
+    L->esize = es;
+    // Plane stride: a multiple of 256 elements plus one 256-element pad, so the
+    // 19 planes do not start at the same offset modulo large powers of two.
+    L->stride = ((ncell + 255) / 256) * 256 + 256;
+    if (stream) {
+        L->stream = (hipStream_t)stream;
+    } else {
+        if (hipStreamCreateWithFlags(&L->own_stream, hipStreamNonBlocking) != hipSuccess) {
+            delete L;
+            return fail(D3Q19_EHIP, "hipStreamCreate failed");
+        }
+        L->stream = L->own_stream;
+    }
+    const size_t lat_bytes = (size_t)Q * L->stride * es;
+    const uint32_t nb = nblocks(ncell);
+    if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
+        hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {
+        d3q19_destroy(L);
+        return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
+    }
+    L->bytes = 2 * lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
+    *out = L;
+    return D3Q19_OK;
+}
+
+int d3q19_destroy(d3q19_lattice* L) {
+    if (!L) return D3Q19_OK;
+    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    void* ps[] = {L->buf[0], L->buf[1], L->force, L->brink, L->solid, L->partials, L->dout, L->uscratch};
+    for (void* p : ps)
+        if (p) (void)hipFree(p);
+    if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
+    delete L;
+    return D3Q19_OK;
+}
+
+int d3q19_set_stream(d3q19_lattice* L, void* stream) {
+    if (int e = check_lat(L)) return e;
+    L->stream = stream ? (hipStream_t)stream : L->own_stream;
+    if (!L->stream) {
+        if (hipStreamCreateWithFlags(&L->own_stream, hipStreamNonBlocking) != hipSuccess)
+            return fail(D3Q19_EHIP, "hipStreamCreate failed");
+        L->stream = L->own_stream;
+    }
+    return D3Q19_OK;
+}
+
+int d3q19_set_populations(d3q19_lattice* L, const void* f, int32_t dt, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (int e = check_dtype(dt)) return e;
+    if (!f) return fail(D3Q19_EINVAL, "null populations");
+    if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
+        return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
+    const size_t n = (size_t)Q * L->ncell;
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    DevBuf tmp;
+    const void* src = f;
+    if (!on_device) {
+        if (tmp.alloc(n * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+        HIPCHK(hipMemcpyAsync(tmp.p, f, n * es, hipMemcpyHostToDevice, L->stream));
+        src = tmp.p;
+    }
+    int rc;
+    if (dt == D3Q19_F64)
+        rc = D3Q19_DISPATCH(L, launch_init, L, (const double*)src);
+    else
+        rc = D3Q19_DISPATCH(L, launch_init, L, (const float*)src);
+    if (rc) return rc;
+    if (!on_device) HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_set_equilibrium(d3q19_lattice* L, const void* rho, const void* u, int32_t dt,
+                          int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (int e = check_dtype(dt)) return e;
+    if (!u) return fail(D3Q19_EINVAL, "null velocity");
+    const size_t nc = L->ncell;
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    DevBuf drho, du, df;
+    const void* prho = rho;
+    const void* pu = u;
+    if (!on_device) {
+        if (du.alloc(3 * nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+        HIPCHK(hipMemcpyAsync(du.p, u, 3 * nc * es, hipMemcpyHostToDevice, L->stream));
+        pu = du.p;
+        if (rho) {
+            if (drho.alloc(nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+            HIPCHK(hipMemcpyAsync(drho.p, rho, nc * es, hipMemcpyHostToDevice, L->stream));
+            prho = drho.p;
+        }
+    }
+    if (df.alloc((size_t)Q * nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+    if (dt == D3Q19_F64)
+        k_fn_feq<double><<<nblocks(nc), BLOCK, 0, L->stream>>>((const double*)prho, (const double*)pu,
+                                                               (double*)df.p, (uint32_t)nc);
+    else
+        k_fn_feq<float><<<nblocks(nc), BLOCK, 0, L->stream>>>((const float*)prho, (const float*)pu,
+                                                              (float*)df.p, (uint32_t)nc);
+    HIPCHK(hipGetLastError());
+    int rc = d3q19_set_populations(L, df.p, dt, 1);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_set_force_field(d3q19_lattice* L, const void* force, int32_t dt, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (!force) return fail(D3Q19_EINVAL, "null force field");
+    const size_t n = 3 * (size_t)L->ncell;
+    if (!L->force) {
+        if (hipMalloc(&L->force, n * ct_size(L)) != hipSuccess)
+            return fail(D3Q19_ENOMEM, "force field allocation failed");
+        L->bytes += n * ct_size(L);
+    }
+    return stage_to_compute(L, force, dt, on_device, n, L->force);
+}
+
+int d3q19_set_brinkman(d3q19_lattice* L, const void* s, int32_t dt, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (!s) {
+        if (L->brink) {
+            HIPCHK(hipStreamSynchronize(L->stream));
+            (void)hipFree(L->brink);
+            L->brink = nullptr;
+        }
+        return D3Q19_OK;
+    }
+    const size_t n = L->ncell;
+    if (!L->brink) {
+        if (hipMalloc(&L->brink, n * ct_size(L)) != hipSuccess)
+            return fail(D3Q19_ENOMEM, "Brinkman field allocation failed");
+        L->bytes += n * ct_size(L);
+    }
+    return stage_to_compute(L, s, dt, on_device, n, L->brink);
+}
+
+int d3q19_set_solid(d3q19_lattice* L, const uint8_t* solid, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (L->cfg.boundary != D3Q19_BC_HALFWAY) return fail(D3Q19_EINVAL, "solid cells need D3Q19_BC_HALFWAY");
+    if (!solid) {
+        if (L->solid) {
+            HIPCHK(hipStreamSynchronize(L->stream));
+            (void)hipFree(L->solid);
+            L->solid = nullptr;
+        }
+        return D3Q19_OK;
+    }
+    if (!L->solid) {
+        if (hipMalloc((void**)&L->solid, L->ncell) != hipSuccess)
+            return fail(D3Q19_ENOMEM, "solid mask allocation failed");
+        L->bytes += L->ncell;
+    }
+    HIPCHK(hipMemcpyAsync(L->solid, solid, L->ncell,
+                          on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_step(d3q19_lattice* L, int32_t n) {
+    if (int e = check_lat(L)) return e;
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state; call set_populations/set_equilibrium");
+    if (n < 0) return fail(D3Q19_EINVAL, "negative step count");
+    if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
+        return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
+    if (n == 0) return D3Q19_OK;
+    return D3Q19_DISPATCH(L, launch_step, L, n, nullptr, nullptr);
+}
+
+int d3q19_step_moments(d3q19_lattice* L, int32_t n, float* rho, float* u) {
+    if (int e = check_lat(L)) return e;
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state; call set_populations/set_equilibrium");
+    if (n < 1) return fail(D3Q19_EINVAL, "step_moments needs n >= 1");
+    if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
+        return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
+    return D3Q19_DISPATCH(L, launch_step, L, n, rho, u);
+}
+
+int d3q19_read_populations(d3q19_lattice* L, void* f, int32_t dt, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (int e = check_dtype(dt)) return e;
+    if (!f) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    const size_t n = (size_t)Q * L->ncell;
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    DevBuf tmp;
+    void* dst = f;
+    if (!on_device) {
+        if (tmp.alloc(n * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+        dst = tmp.p;
+    }
+    int rc = dt == D3Q19_F64
+                 ? D3Q19_DISPATCH(L, launch_readback, L, (double*)dst, (double*)nullptr, (double*)nullptr)
+                 : D3Q19_DISPATCH(L, launch_readback, L, (float*)dst, (float*)nullptr, (float*)nullptr);
+    if (rc) return rc;
+    if (!on_device) HIPCHK(hipMemcpyAsync(f, dst, n * es, hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_read_moments(d3q19_lattice* L, void* rho, void* u, int32_t dt, int32_t on_device) {
+    if (int e = check_lat(L)) return e;
+    if (int e = check_dtype(dt)) return e;
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (!rho && !u) return D3Q19_OK;
+    const size_t nc = L->ncell;
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    DevBuf trho, tu;
+    void* drho = rho;
+    void* du = u;
+    if (!on_device) {
+        if (rho) {
+            if (trho.alloc(nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+            drho = trho.p;
+        }
+        if (u) {
+            if (tu.alloc(3 * nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
+            du = tu.p;
+        }
+    }
+    int rc = dt == D3Q19_F64
+                 ? D3Q19_DISPATCH(L, launch_readback, L, (double*)nullptr, (double*)drho, (double*)du)
+                 : D3Q19_DISPATCH(L, launch_readback, L, (float*)nullptr, (float*)drho, (float*)du);
+    if (rc) return rc;
+    if (!on_device) {
+        if (rho) HIPCHK(hipMemcpyAsync(rho, drho, nc * es, hipMemcpyDeviceToHost, L->stream));
+        if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * nc * es, hipMemcpyDeviceToHost, L->stream));
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_diagnostics(d3q19_lattice* L, double out[5]) {
+    if (int e = check_lat(L)) return e;
+    if (!out) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    int rc = D3Q19_DISPATCH(L, launch_diag, L);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(out, L->dout, NDIAG * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_enstrophy(d3q19_lattice* L, double* out) {
+    if (int e = check_lat(L)) return e;
+    if (!out) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (!L->uscratch) {
+        if (hipMalloc((void**)&L->uscratch, 3 * (size_t)L->ncell * sizeof(float)) != hipSuccess)
+            return fail(D3Q19_ENOMEM, "velocity scratch allocation failed");
+        L->bytes += 3 * (size_t)L->ncell * sizeof(float);
+    }
+    int rc = D3Q19_DISPATCH(L, launch_readback, L, (float*)nullptr, (float*)nullptr, L->uscratch);
+    if (rc) return rc;
+    const uint32_t g = nblocks(L->ncell);
+    k_enstrophy<<<g, BLOCK, 0, L->stream>>>(L->uscratch, L->cfg.nx, L->cfg.ny, L->cfg.nz, L->partials);
+    HIPCHK(hipGetLastError());
+    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, g, 1, L->dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, L->dout, sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_synchronize(d3q19_lattice* L) {
+    if (int e = check_lat(L)) return e;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int64_t d3q19_step_count(const d3q19_lattice* L) { return L ? L->steps : -1; }
+size_t d3q19_device_bytes(const d3q19_lattice* L) { return L ? L->bytes : 0; }
+
+// ---------------------------------------------------------------------------
+// Stateless drop-ins.
+// ---------------------------------------------------------------------------
+static int check_fn(const void* a, const void* b, int64_t ncell, int32_t dt) {
+    if (!a || !b) return fail(D3Q19_EINVAL, "null array");
+    if (ncell < 1 || (uint64_t)ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "bad cell count");
+    return check_dtype(dt);
+}
+
+int d3q19_bgk_step(const void* f, void* out, const void* force, int32_t nx, int32_t ny, int32_t nz,
+                   double tau, int32_t dt, void* stream) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
+    const int64_t nc = (int64_t)nx * ny * nz;
+    if (int e = check_fn(f, out, nc, dt)) return e;
+    if (f == out) return fail(D3Q19_EINVAL, "bgk_step cannot run in place");
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_bgk_step<double><<<nblocks(nc), BLOCK, 0, s>>>((const double*)f, (double*)out,
+                                                             (const double*)force, nx, ny, nz, tau);
+    else
+        k_fn_bgk_step<float><<<nblocks(nc), BLOCK, 0, s>>>((const float*)f, (float*)out,
+                                                            (const float*)force, nx, ny, nz, (float)tau);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int d3q19_stream(const void* f, void* out, int32_t nx, int32_t ny, int32_t nz, int32_t dt, void* stream) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
+    const int64_t nc = (int64_t)nx * ny * nz;
+    if (int e = check_fn(f, out, nc, dt)) return e;
+    if (f == out) return fail(D3Q19_EINVAL, "stream cannot run in place");
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_stream<double><<<nblocks(nc), BLOCK, 0, s>>>((const double*)f, (double*)out, nx, ny, nz);
+    else
+        k_fn_stream<float><<<nblocks(nc), BLOCK, 0, s>>>((const float*)f, (float*)out, nx, ny, nz);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int d3q19_bounce_back_y_walls(const void* f, void* out, const double top[3], const double bot[3],
+                              int32_t nx, int32_t ny, int32_t nz, int32_t dt, void* stream) {
+    if (nx < 1 || ny < 1 || nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
+    const int64_t nc = (int64_t)nx * ny * nz;
+    if (int e = check_fn(f, out, nc, dt)) return e;
+    if (f == out) return fail(D3Q19_EINVAL, "bounce-back cannot run in place");
+    const double zero[3] = {0, 0, 0};
+    if (!top) top = zero;
+    if (!bot) bot = zero;
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_bb_ywalls<double><<<nblocks(nc), BLOCK, 0, s>>>((const double*)f, (double*)out, nx, ny, nz,
+                                                              top[0], top[1], top[2], bot[0], bot[1], bot[2]);
+    else
+        k_fn_bb_ywalls<float><<<nblocks(nc), BLOCK, 0, s>>>(
+            (const float*)f, (float*)out, nx, ny, nz, (float)top[0], (float)top[1], (float)top[2],
+            (float)bot[0], (float)bot[1], (float)bot[2]);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int d3q19_feq_field(const void* rho, const void* u, void* out, int64_t nc, int32_t dt, void* stream) {
+    if (int e = check_fn(u, out, nc, dt)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_feq<double><<<nblocks(nc), BLOCK, 0, s>>>((const double*)rho, (const double*)u, (double*)out,
+                                                        (uint32_t)nc);
+    else
+        k_fn_feq<float><<<nblocks(nc), BLOCK, 0, s>>>((const float*)rho, (const float*)u, (float*)out,
+                                                       (uint32_t)nc);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+static int fn_moments(const void* f, const void* force, void* rho, void* mom, void* u, int64_t nc,
+                      int32_t dt, void* stream) {
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_moments<double><<<nblocks(nc), BLOCK, 0, s>>>((const double*)f, (const double*)force,
+                                                            (double*)rho, (double*)mom, (double*)u,
+                                                            (uint32_t)nc);
+    else
+        k_fn_moments<float><<<nblocks(nc), BLOCK, 0, s>>>((const float*)f, (const float*)force,
+                                                           (float*)rho, (float*)mom, (float*)u, (uint32_t)nc);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int d3q19_density_field(const void* f, void* rho, int64_t nc, int32_t dt, void* stream) {
+    if (int e = check_fn(f, rho, nc, dt)) return e;
+    return fn_moments(f, nullptr, rho, nullptr, nullptr, nc, dt, stream);
+}
+
+int d3q19_momentum_field(const void* f, void* mom, int64_t nc, int32_t dt, void* stream) {
+    if (int e = check_fn(f, mom, nc, dt)) return e;
+    return fn_moments(f, nullptr, nullptr, mom, nullptr, nc, dt, stream);
+}
+
+int d3q19_macroscopic_velocity(const void* f, const void* force, void* u, int64_t nc, int32_t dt,
+                               void* stream) {
+    if (int e = check_fn(f, u, nc, dt)) return e;
+    return fn_moments(f, force, nullptr, nullptr, u, nc, dt, stream);
+}
+
+}  // extern "C"

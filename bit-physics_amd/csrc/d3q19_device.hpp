@@ -1,0 +1,318 @@
+// d3q19_device.hpp -- device-side building blocks of the D3Q19 engine (gfx950).
+//
+// Lattice: lexicographic D3Q19 velocity set of the reference
+// (packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/constants.py:39-69),
+// opposite map of reference/bgk.py:164-187.
+//
+// Populations live on the GPU as deviations d_i = f_i - w_i ("shifted"
+// storage), in one of four storage formats; all collision arithmetic is in
+// the compute type C (float, or double for FP64 storage).  Storing the
+// deviation keeps the O(1e-3) signal instead of the O(0.1) rest weight in the
+// mantissa, which is what makes 16-bit storage usable and FP32 storage
+// accurate to ~1e-10 absolute per step.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace d3q19 {
+
+constexpr int Q = 19;
+constexpr int CX[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+constexpr int CY[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+constexpr int CZ[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+constexpr int OPP[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+
+template <class T>
+__host__ __device__ constexpr T wgt(int i) {
+    return i == 0 ? T(1) / T(3) : (i <= 6 ? T(1) / T(18) : T(1) / T(36));
+}
+
+enum { BC_PERIODIC = 0, BC_YWALLS = 1, BC_HALFWAY = 2 };
+enum { FORCE_NONE = 0, FORCE_UNIFORM = 1, FORCE_FIELD = 2 };
+
+// ---------------------------------------------------------------------------
+// Storage formats.  raw = element type in HBM, C = compute type.
+// ---------------------------------------------------------------------------
+struct TF64 {};
+struct TF32 {};
+struct TF16 {};   // IEEE half of (f_i - w_i) * (2048 / w_i)
+struct TBF16 {};  // bfloat16 of (f_i - w_i)
+
+template <class Tag> struct Codec;
+
+template <> struct Codec<TF64> {
+    typedef double raw;
+    typedef double C;
+    __device__ static __forceinline__ double dec(raw v, int) { return v; }
+    __device__ static __forceinline__ raw enc(double d, int) { return d; }
+};
+
+template <> struct Codec<TF32> {
+    typedef float raw;
+    typedef float C;
+    __device__ static __forceinline__ float dec(raw v, int) { return v; }
+    __device__ static __forceinline__ raw enc(float d, int) { return d; }
+};
+
+// Half: the deviation relative to w_i, times 2^11, so that relative
+// deviations from 3e-8 to 32 stay in the normal FP16 range.
+template <> struct Codec<TF16> {
+    typedef uint16_t raw;
+    typedef float C;
+    __host__ __device__ static constexpr float scale(int i) {
+        return i == 0 ? 3.0f * 2048.0f : (i <= 6 ? 18.0f * 2048.0f : 36.0f * 2048.0f);
+    }
+    __device__ static __forceinline__ float dec(raw v, int i) {
+        _Float16 h = __builtin_bit_cast(_Float16, v);
+        return (float)h * (1.0f / scale(i));
+    }
+    __device__ static __forceinline__ raw enc(float d, int i) {
+        _Float16 h = (_Float16)(d * scale(i));  // round to nearest even
+        return __builtin_bit_cast(uint16_t, h);
+    }
+};
+
+template <> struct Codec<TBF16> {
+    typedef uint16_t raw;
+    typedef float C;
+    __device__ static __forceinline__ float dec(raw v, int) {
+        return __builtin_bit_cast(float, (uint32_t)v << 16);
+    }
+    __device__ static __forceinline__ raw enc(float d, int) {
+        uint32_t b = __builtin_bit_cast(uint32_t, d);
+        if ((b & 0x7f800000u) == 0x7f800000u && (b & 0x007fffffu)) return (raw)0x7fc0u;  // NaN
+        b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
+        return (raw)(b >> 16);
+    }
+};
+
+// Per-direction base pointers: population i of cell c is base[i][c].  Each
+// base is its own plane, so offsets stay 32-bit even when the lattice is
+// larger than 4 GiB.
+template <class Tag> struct Planes {
+    typename Codec<Tag>::raw* p[Q];
+};
+template <class Tag> struct CPlanes {
+    const typename Codec<Tag>::raw* p[Q];
+};
+
+template <class C> struct Params {
+    uint32_t nx, ny, nz, ncell;
+    C omega;  // 1 / tau
+    C pref;   // 1 - 1 / (2 tau)   (Guo prefactor, reference/bgk.py:90)
+    int32_t force_mode;
+    uint32_t wall_mask;
+    C fu[3];                // uniform force
+    const C* force;         // (3, ncell) field or null
+    const C* brink;         // (ncell) Brinkman solid fraction or null
+    const uint8_t* solid;   // (ncell) solid flag or null
+    C face_u[6][3];         // half-way wall velocities
+    C wtop[3], wbot[3];     // reference y-wall velocities
+};
+
+template <class T>
+__device__ __forceinline__ T ldg(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+// ---------------------------------------------------------------------------
+// Pull streaming with boundary handling.  Produces the pre-collision
+// deviations d_i(x) = f_i(x) - w_i of the reference state after streaming and
+// bounce-back, from the stored post-collision deviations.
+// ---------------------------------------------------------------------------
+template <class Tag, int BC>
+__device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typename Codec<Tag>::C>& p,
+                                     uint32_t x, uint32_t y, uint32_t z, uint32_t cell,
+                                     typename Codec<Tag>::C d[Q]) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t nyz = p.ny * p.nz;
+    const uint32_t xm = (x == 0 ? p.nx - 1 : x - 1) * nyz, x0 = x * nyz,
+                   xp = (x + 1 == p.nx ? 0 : x + 1) * nyz;
+    const uint32_t ym = (y == 0 ? p.ny - 1 : y - 1) * p.nz, y0 = y * p.nz,
+                   yp = (y + 1 == p.ny ? 0 : y + 1) * p.nz;
+    const uint32_t zm = (z == 0 ? p.nz - 1 : z - 1), z0 = z, zp = (z + 1 == p.nz ? 0 : z + 1);
+    if constexpr (BC != BC_HALFWAY) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
+            const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
+            const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
+            d[i] = Codec<Tag>::dec(in.p[i][sx + sy + sz], i);
+        }
+    }
+    if constexpr (BC == BC_YWALLS) {
+        // reference/bgk.py:190-215 on the post-stream values, in registers.
+        const bool bot = (y == 0), top = (y + 1 == p.ny);
+        if (bot || top) {
+            C rw = C(0);
+#pragma unroll
+            for (int i = 0; i < Q; ++i) rw += d[i];
+            rw += C(1);
+            C nd[Q];
+#pragma unroll
+            for (int i = 0; i < Q; ++i) nd[i] = d[i];
+            if (bot) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i)
+                    if (CY[i] > 0) {
+                        C cu = C(CX[i]) * p.wbot[0] + C(CY[i]) * p.wbot[1] + C(CZ[i]) * p.wbot[2];
+                        nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
+                    }
+            }
+            if (top) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i)
+                    if (CY[i] < 0) {
+                        C cu = C(CX[i]) * p.wtop[0] + C(CY[i]) * p.wtop[1] + C(CZ[i]) * p.wtop[2];
+                        nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
+                    }
+            }
+#pragma unroll
+            for (int i = 0; i < Q; ++i) d[i] = nd[i];
+        }
+    }
+    if constexpr (BC == BC_HALFWAY) {
+        const uint32_t wm = p.wall_mask;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            int face = -1;
+            if (CX[i] > 0 && x == 0 && (wm & 1u)) face = 0;
+            if (CX[i] < 0 && x + 1 == p.nx && (wm & 2u)) face = 1;
+            if (CY[i] > 0 && y == 0 && (wm & 4u)) face = 2;
+            if (CY[i] < 0 && y + 1 == p.ny && (wm & 8u)) face = 3;
+            if (CZ[i] > 0 && z == 0 && (wm & 16u)) face = 4;
+            if (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u)) face = 5;
+            if (face >= 0) {
+                C cu = C(CX[i]) * p.face_u[face][0] + C(CY[i]) * p.face_u[face][1] +
+                       C(CZ[i]) * p.face_u[face][2];
+                d[i] = Codec<Tag>::dec(in.p[OPP[i]][cell], OPP[i]) + C(6) * wgt<C>(i) * cu;
+            } else {
+                const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
+                const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
+                const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
+                const uint32_t src = sx + sy + sz;
+                if (p.solid && p.solid[src])
+                    d[i] = Codec<Tag>::dec(in.p[OPP[i]][cell], OPP[i]);
+                else
+                    d[i] = Codec<Tag>::dec(in.p[i][src], i);
+            }
+        }
+    }
+}
+
+// Moments of the deviations: drho = rho - 1, m = sum c_i f_i.
+template <class C>
+__device__ __forceinline__ void moments(const C d[Q], C& drho, C& mx, C& my, C& mz) {
+    C r = C(0), a = C(0), b = C(0), c = C(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        r += d[i];
+        if (CX[i] > 0) a += d[i];
+        if (CX[i] < 0) a -= d[i];
+        if (CY[i] > 0) b += d[i];
+        if (CY[i] < 0) b -= d[i];
+        if (CZ[i] > 0) c += d[i];
+        if (CZ[i] < 0) c -= d[i];
+    }
+    drho = r;
+    mx = a;
+    my = b;
+    mz = c;
+}
+
+// Body force at a cell (Guo), including the Brinkman drag -s * m.
+template <class C>
+__device__ __forceinline__ void cell_force(const Params<C>& p, uint32_t cell, C mx, C my, C mz,
+                                           C& fx, C& fy, C& fz) {
+    fx = C(0);
+    fy = C(0);
+    fz = C(0);
+    if (p.force_mode == FORCE_UNIFORM) {
+        fx = p.fu[0];
+        fy = p.fu[1];
+        fz = p.fu[2];
+    } else if (p.force_mode == FORCE_FIELD) {
+        fx = p.force[cell];
+        fy = p.force[p.ncell + cell];
+        fz = p.force[2 * p.ncell + cell];
+    }
+    if (p.brink) {
+        const C s = p.brink[cell];
+        fx -= s * mx;
+        fy -= s * my;
+        fz -= s * mz;
+    }
+}
+
+// BGK collision with Guo forcing on deviations (reference/bgk.py:72-110):
+//   u = (m + F/2)/rho,  f*_i = f_i - (f_i - feq_i(rho,u))/tau + G_i
+//   feq_i - w_i = w_i (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
+//   G_i = (1 - 1/(2 tau)) w_i [3 (c_i - u).F + 9 (c_i.u)(c_i.F)]
+// Returns rho and the Guo-corrected velocity (macroscopic_velocity, bgk.py:129-142).
+template <class C>
+__device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cell, C& rho_o,
+                                        C& ux_o, C& uy_o, C& uz_o) {
+    C drho, mx, my, mz;
+    moments(d, drho, mx, my, mz);
+    const C rho = C(1) + drho;
+    C fx, fy, fz;
+    cell_force(p, cell, mx, my, mz, fx, fy, fz);
+    const bool forced = (p.force_mode != FORCE_NONE) || (p.brink != nullptr);
+    const C rs = rho > C(1e-30) ? rho : C(1e-30);
+    const C irs = C(1) / rs;
+    const C ux = (mx + C(0.5) * fx) * irs;
+    const C uy = (my + C(0.5) * fy) * irs;
+    const C uz = (mz + C(0.5) * fz) * irs;
+    const C usq15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
+    const C om = p.omega;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
+        const C deq = wgt<C>(i) * (drho + rho * (C(3) * cu + C(4.5) * cu * cu - usq15));
+        C v = d[i] + om * (deq - d[i]);
+        if (forced) {
+            const C cf = C(CX[i]) * fx + C(CY[i]) * fy + C(CZ[i]) * fz;
+            const C uf = ux * fx + uy * fy + uz * fz;
+            v += p.pref * wgt<C>(i) * (C(3) * (cf - uf) + C(9) * cu * cf);
+        }
+        d[i] = v;
+    }
+    rho_o = rho;
+    ux_o = ux;
+    uy_o = uy;
+    uz_o = uz;
+}
+
+// Read-back moments of pre-collision deviations (no collision).
+template <class C>
+__device__ __forceinline__ void readback_moments(const C d[Q], const Params<C>& p, uint32_t cell,
+                                                 C& rho_o, C& ux, C& uy, C& uz) {
+    C drho, mx, my, mz;
+    moments(d, drho, mx, my, mz);
+    const C rho = C(1) + drho;
+    C fx, fy, fz;
+    cell_force(p, cell, mx, my, mz, fx, fy, fz);
+    const C rs = rho > C(1e-30) ? rho : C(1e-30);
+    ux = (mx + C(0.5) * fx) / rs;
+    uy = (my + C(0.5) * fy) / rs;
+    uz = (mz + C(0.5) * fz) / rs;
+    rho_o = rho;
+}
+
+__device__ __forceinline__ void cell_coords(uint32_t cell, uint32_t ny, uint32_t nz, uint32_t& x,
+                                            uint32_t& y, uint32_t& z) {
+    z = cell % nz;
+    const uint32_t t = cell / nz;
+    y = t % ny;
+    x = t / ny;
+}
+
+// Wavefront (64-lane) sum of a double.
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+}  // namespace d3q19

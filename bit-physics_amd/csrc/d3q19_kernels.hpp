@@ -1,0 +1,425 @@
+// d3q19_kernels.hpp -- HIP kernels of the D3Q19 engine (gfx950).
+#pragma once
+
+#include "d3q19_device.hpp"
+
+namespace d3q19 {
+
+constexpr int BLOCK = 256;
+
+// ---------------------------------------------------------------------------
+// The hot path: one fused pull-stream / boundary / moments / BGK+Guo pass.
+// Reads the post-collision state of step n, writes that of step n+1, and
+// (optionally) rho and u of the reference state f^{n+1} from the same pass.
+// ---------------------------------------------------------------------------
+template <class Tag, int BC>
+static __global__ void __launch_bounds__(BLOCK)
+    k_step(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
+           float* u_out) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= p.ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, p.ny, p.nz, x, y, z);
+    if constexpr (BC == BC_HALFWAY) {
+        if (p.solid && p.solid[cell]) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                __builtin_nontemporal_store(Codec<Tag>::enc(C(0), i), &out.p[i][cell]);
+            if (rho_out) rho_out[cell] = 1.0f;
+            if (u_out) {
+                u_out[cell] = 0.0f;
+                u_out[p.ncell + cell] = 0.0f;
+                u_out[2 * p.ncell + cell] = 0.0f;
+            }
+            return;
+        }
+    }
+    C d[Q];
+    pull<Tag, BC>(in, p, x, y, z, cell, d);
+    C rho, ux, uy, uz;
+    collide(d, p, cell, rho, ux, uy, uz);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) __builtin_nontemporal_store(Codec<Tag>::enc(d[i], i), &out.p[i][cell]);
+    if (rho_out) rho_out[cell] = (float)rho;
+    if (u_out) {
+        u_out[cell] = (float)ux;
+        u_out[p.ncell + cell] = (float)uy;
+        u_out[2 * p.ncell + cell] = (float)uz;
+    }
+}
+
+// Collide a reference-layout population field f^n (no streaming) into the
+// engine's post-collision storage: the state the fused step consumes.
+template <class Tag, class In>
+static __global__ void __launch_bounds__(BLOCK)
+    k_init(const In* __restrict__ f, Planes<Tag> out, Params<typename Codec<Tag>::C> p) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= p.ncell) return;
+    C d[Q];
+    if (p.solid && p.solid[cell]) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) out.p[i][cell] = Codec<Tag>::enc(C(0), i);
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        d[i] = (C)((double)f[(size_t)i * p.ncell + cell] - wgt<double>(i));
+    C rho, ux, uy, uz;
+    collide(d, p, cell, rho, ux, uy, uz);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out.p[i][cell] = Codec<Tag>::enc(d[i], i);
+}
+
+// Read-back: pull + boundary of the stored post-collision state gives the
+// reference's pre-collision f^n; write it and/or its moments.
+template <class Tag, int BC, class Out>
+static __global__ void __launch_bounds__(BLOCK)
+    k_readback(CPlanes<Tag> in, Params<typename Codec<Tag>::C> p, Out* __restrict__ f_out,
+               Out* __restrict__ rho_out, Out* __restrict__ u_out) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= p.ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, p.ny, p.nz, x, y, z);
+    C d[Q];
+    const bool solid = (BC == BC_HALFWAY) && p.solid && p.solid[cell];
+    if (solid) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d[i] = C(0);
+    } else {
+        pull<Tag, BC>(in, p, x, y, z, cell, d);
+    }
+    if (f_out) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) f_out[(size_t)i * p.ncell + cell] = (Out)(wgt<double>(i) + (double)d[i]);
+    }
+    if (rho_out || u_out) {
+        C rho, ux, uy, uz;
+        readback_moments(d, p, cell, rho, ux, uy, uz);
+        if (solid) ux = uy = uz = C(0);
+        if (rho_out) rho_out[cell] = (Out)rho;
+        if (u_out) {
+            u_out[cell] = (Out)ux;
+            u_out[p.ncell + cell] = (Out)uy;
+            u_out[2 * p.ncell + cell] = (Out)uz;
+        }
+    }
+}
+
+// FP64 diagnostics of f^n: per-block partial sums of
+// [mass, px, py, pz, kinetic energy], summed in a fixed order afterwards.
+constexpr int NDIAG = 5;
+
+template <class Tag, int BC>
+static __global__ void __launch_bounds__(BLOCK)
+    k_diag(CPlanes<Tag> in, Params<typename Codec<Tag>::C> p, double* __restrict__ partials) {
+    typedef typename Codec<Tag>::C C;
+    __shared__ double red[BLOCK / 64][NDIAG];
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    double v[NDIAG] = {0, 0, 0, 0, 0};
+    if (cell < p.ncell && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
+        uint32_t x, y, z;
+        cell_coords(cell, p.ny, p.nz, x, y, z);
+        C d[Q];
+        pull<Tag, BC>(in, p, x, y, z, cell, d);
+        C drho, mx, my, mz, rho, ux, uy, uz;
+        moments(d, drho, mx, my, mz);
+        readback_moments(d, p, cell, rho, ux, uy, uz);
+        v[0] = 1.0 + (double)drho;
+        v[1] = (double)mx;
+        v[2] = (double)my;
+        v[3] = (double)mz;
+        v[4] = 0.5 * (double)rho * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NDIAG; ++k) {
+        double s = wave_sum(v[k]);
+        if (lane == 0) red[wave][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NDIAG) {
+        double s = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) s += red[w][threadIdx.x];
+        partials[(size_t)blockIdx.x * NDIAG + threadIdx.x] = s;
+    }
+}
+
+// Fixed-order final sum of per-block partials (one block).
+static __global__ void __launch_bounds__(BLOCK)
+    k_sum_partials(const double* __restrict__ partials, uint32_t nblocks, int ncomp,
+                   double* __restrict__ out) {
+    __shared__ double red[BLOCK / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k = 0; k < ncomp; ++k) {
+        double s = 0.0;
+        for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) s += partials[(size_t)b * ncomp + k];
+        s = wave_sum(s);
+        if (lane == 0) red[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
+// FP64 enstrophy partials 0.5 |curl u|^2 with periodic central differences.
+static __global__ void __launch_bounds__(BLOCK)
+    k_enstrophy(const float* __restrict__ u, uint32_t nx, uint32_t ny, uint32_t nz,
+                double* __restrict__ partials) {
+    __shared__ double red[BLOCK / 64];
+    const uint32_t ncell = nx * ny * nz;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (cell < ncell) {
+        uint32_t x, y, z;
+        cell_coords(cell, ny, nz, x, y, z);
+        const uint32_t nyz = ny * nz;
+        const uint32_t xm = (x == 0 ? nx - 1 : x - 1), xp = (x + 1 == nx ? 0 : x + 1);
+        const uint32_t ym = (y == 0 ? ny - 1 : y - 1), yp = (y + 1 == ny ? 0 : y + 1);
+        const uint32_t zm = (z == 0 ? nz - 1 : z - 1), zp = (z + 1 == nz ? 0 : z + 1);
+        const float* ux = u;
+        const float* uy = u + ncell;
+        const float* uz = u + 2 * (size_t)ncell;
+        auto at = [&](const float* a, uint32_t i, uint32_t j, uint32_t k) {
+            return (double)a[i * nyz + j * nz + k];
+        };
+        const double dwy_dz = 0.5 * (at(uy, x, y, zp) - at(uy, x, y, zm));
+        const double dwz_dy = 0.5 * (at(uz, x, yp, z) - at(uz, x, ym, z));
+        const double dwx_dz = 0.5 * (at(ux, x, y, zp) - at(ux, x, y, zm));
+        const double dwz_dx = 0.5 * (at(uz, xp, y, z) - at(uz, xm, y, z));
+        const double dwy_dx = 0.5 * (at(uy, xp, y, z) - at(uy, xm, y, z));
+        const double dwx_dy = 0.5 * (at(ux, x, yp, z) - at(ux, x, ym, z));
+        const double ox = dwz_dy - dwy_dz, oy = dwx_dz - dwz_dx, oz = dwy_dx - dwx_dy;
+        v = 0.5 * (ox * ox + oy * oy + oz * oz);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double s = wave_sum(v);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+        partials[blockIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stateless drop-ins on reference-layout arrays (unshifted populations).
+// FP64 instantiations follow the NumPy reference's operation order term by
+// term with contraction off, so they agree with it to the last bit.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void ref_moments(const T fc[Q], T& r, T& mx, T& my, T& mz) {
+#pragma clang fp contract(off)
+    T a = T(0), bx = T(0), by = T(0), bz = T(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        a = a + fc[i];
+        bx = bx + T(CX[i]) * fc[i];
+        by = by + T(CY[i]) * fc[i];
+        bz = bz + T(CZ[i]) * fc[i];
+    }
+    r = a;
+    mx = bx;
+    my = by;
+    mz = bz;
+}
+
+template <class T> struct RefConst {
+    T cs2, inv_cs2, inv_cs4, inv_two_cs4, inv_two_cs2;
+};
+
+template <class T>
+__host__ __device__ inline RefConst<T> ref_const() {
+    RefConst<T> k;
+    k.cs2 = T(1) / T(3);
+    k.inv_cs2 = T(1) / k.cs2;
+    k.inv_cs4 = T(1) / (k.cs2 * k.cs2);
+    k.inv_two_cs4 = T(1) / (T(2) * k.cs2 * k.cs2);
+    k.inv_two_cs2 = T(1) / (T(2) * k.cs2);
+    return k;
+}
+
+template <class T>
+__device__ __forceinline__ T rho_floor(T r) {
+    return (r != r) ? r : (r > T(1e-30) ? r : T(1e-30));
+}
+
+// reference/equilibrium.py:90-98, one direction
+template <class T>
+__device__ __forceinline__ T ref_feq(int i, T r, T ux, T uy, T uz, T u_sq, const RefConst<T>& k) {
+#pragma clang fp contract(off)
+    T cu = T(CX[i]) * ux + T(CY[i]) * uy + T(CZ[i]) * uz;
+    return wgt<T>(i) * r * (T(1) + cu * k.inv_cs2 + cu * cu * k.inv_two_cs4 - u_sq * k.inv_two_cs2);
+}
+
+// reference/bgk.py:72-110 collision of one cell.
+template <class T>
+__device__ __forceinline__ void ref_collide(T fc[Q], T tau, bool has_force, T fx, T fy, T fz) {
+#pragma clang fp contract(off)
+    const RefConst<T> k = ref_const<T>();
+    T r, mx, my, mz;
+    ref_moments(fc, r, mx, my, mz);
+    T rs = rho_floor(r);
+    T ux = mx / rs, uy = my / rs, uz = mz / rs;
+    if (has_force) {
+        ux = ux + T(0.5) * fx / rs;
+        uy = uy + T(0.5) * fy / rs;
+        uz = uz + T(0.5) * fz / rs;
+    }
+    T u_sq = ux * ux + uy * uy + uz * uz;
+    T pref = T(1) - T(0.5) / tau;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        T feq = ref_feq(i, r, ux, uy, uz, u_sq, k);
+        T fp = fc[i] - (fc[i] - feq) / tau;
+        if (has_force) {
+            T c0 = T(CX[i]), c1 = T(CY[i]), c2 = T(CZ[i]);
+            T cdu = c0 * ux + c1 * uy + c2 * uz;
+            T tx = (c0 - ux) * k.inv_cs2 + cdu * c0 * k.inv_cs4;
+            T ty = (c1 - uy) * k.inv_cs2 + cdu * c1 * k.inv_cs4;
+            T tz = (c2 - uz) * k.inv_cs2 + cdu * c2 * k.inv_cs4;
+            T ft = pref * wgt<T>(i) * (tx * fx + ty * fy + tz * fz);
+            fp = fp + ft;
+        }
+        fc[i] = fp;
+    }
+}
+
+// bgk_step: collide, then push along c_i (== np.roll by +c_i).
+template <class T>
+static __global__ void __launch_bounds__(BLOCK)
+    k_fn_bgk_step(const T* __restrict__ f, T* __restrict__ out, const T* __restrict__ force,
+                  uint32_t nx, uint32_t ny, uint32_t nz, T tau) {
+    const uint32_t ncell = nx * ny * nz;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, ny, nz, x, y, z);
+    T fc[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fc[i] = f[(size_t)i * ncell + cell];
+    T fx = T(0), fy = T(0), fz = T(0);
+    if (force) {
+        fx = force[cell];
+        fy = force[ncell + cell];
+        fz = force[2 * (size_t)ncell + cell];
+    }
+    ref_collide(fc, tau, force != nullptr, fx, fy, fz);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const uint32_t dx = (x + nx + CX[i]) % nx, dy = (y + ny + CY[i]) % ny, dz = (z + nz + CZ[i]) % nz;
+        out[(size_t)i * ncell + (dx * ny + dy) * nz + dz] = fc[i];
+    }
+}
+
+// stream: out[i, x] = in[i, x - c_i] (periodic).
+template <class T>
+static __global__ void __launch_bounds__(BLOCK)
+    k_fn_stream(const T* __restrict__ f, T* __restrict__ out, uint32_t nx, uint32_t ny, uint32_t nz) {
+    const uint32_t ncell = nx * ny * nz;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, ny, nz, x, y, z);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const uint32_t sx = (x + nx - CX[i]) % nx, sy = (y + ny - CY[i]) % ny, sz = (z + nz - CZ[i]) % nz;
+        out[(size_t)i * ncell + cell] = f[(size_t)i * ncell + (sx * ny + sy) * nz + sz];
+    }
+}
+
+// apply_bounce_back_y_walls (reference/bgk.py:145-216).
+template <class T>
+static __global__ void __launch_bounds__(BLOCK)
+    k_fn_bb_ywalls(const T* __restrict__ f, T* __restrict__ out, uint32_t nx, uint32_t ny,
+                   uint32_t nz, T t0, T t1, T t2, T b0, T b1, T b2) {
+#pragma clang fp contract(off)
+    const uint32_t ncell = nx * ny * nz;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, ny, nz, x, y, z);
+    T fc[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fc[i] = f[(size_t)i * ncell + cell];
+    T oc[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) oc[i] = fc[i];
+    const T cs2 = T(1) / T(3);
+    if (y == 0 || y + 1 == ny) {
+        T rw = T(0);
+#pragma unroll
+        for (int d = 0; d < Q; ++d) rw = rw + fc[d];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (CY[i] > 0 && y == 0) {
+                T cdu = T(CX[i]) * b0 + T(CY[i]) * b1 + T(CZ[i]) * b2;
+                T inj = T(-2) * wgt<T>(i) * rw * cdu / cs2;
+                oc[i] = fc[OPP[i]] + inj;
+            }
+            if (CY[i] < 0 && y + 1 == ny) {
+                T cdu = T(CX[i]) * t0 + T(CY[i]) * t1 + T(CZ[i]) * t2;
+                T inj = T(-2) * wgt<T>(i) * rw * cdu / cs2;
+                oc[i] = fc[OPP[i]] + inj;
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out[(size_t)i * ncell + cell] = oc[i];
+}
+
+// feq_field (reference/equilibrium.py:71-98); rho == nullptr means rho = 1.
+template <class T>
+static __global__ void __launch_bounds__(BLOCK)
+    k_fn_feq(const T* __restrict__ rho, const T* __restrict__ u, T* __restrict__ out, uint32_t ncell) {
+#pragma clang fp contract(off)
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= ncell) return;
+    const RefConst<T> k = ref_const<T>();
+    const T ux = u[cell], uy = u[ncell + cell], uz = u[2 * (size_t)ncell + cell];
+    const T u_sq = ux * ux + uy * uy + uz * uz;
+    const T r = rho ? rho[cell] : T(1);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) out[(size_t)i * ncell + cell] = ref_feq(i, r, ux, uy, uz, u_sq, k);
+}
+
+// density_field / momentum_field / macroscopic_velocity; any output may be null.
+template <class T>
+static __global__ void __launch_bounds__(BLOCK)
+    k_fn_moments(const T* __restrict__ f, const T* __restrict__ force, T* __restrict__ rho_out,
+                 T* __restrict__ mom_out, T* __restrict__ u_out, uint32_t ncell) {
+#pragma clang fp contract(off)
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= ncell) return;
+    T fc[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fc[i] = f[(size_t)i * ncell + cell];
+    T r, mx, my, mz;
+    ref_moments(fc, r, mx, my, mz);
+    if (rho_out) rho_out[cell] = r;
+    if (mom_out) {
+        mom_out[cell] = mx;
+        mom_out[ncell + cell] = my;
+        mom_out[2 * (size_t)ncell + cell] = mz;
+    }
+    if (u_out) {
+        const T rs = rho_floor(r);
+        if (force) {
+            mx = mx + T(0.5) * force[cell];
+            my = my + T(0.5) * force[ncell + cell];
+            mz = mz + T(0.5) * force[2 * (size_t)ncell + cell];
+        }
+        u_out[cell] = mx / rs;
+        u_out[ncell + cell] = my / rs;
+        u_out[2 * (size_t)ncell + cell] = mz / rs;
+    }
+}
+
+}  // namespace d3q19

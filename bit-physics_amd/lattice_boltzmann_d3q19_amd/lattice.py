@@ -1,0 +1,286 @@
+"""Stateful lattice: the step / read-back path as one object over the C ABI.
+
+Replaces the runner loop of the reference
+(packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/sim.py:312-326 and
+:350-364)::
+
+    f = bgk_step(f, tau, force_lattice=F)
+    f = apply_bounce_back_y_walls(f, wall_velocity_top=..., wall_velocity_bottom=...)
+
+with ``Lattice.step(n)``: one fused pull-stream / bounce-back / BGK+Guo
+kernel per step on the GPU.  ``populations()`` returns the reference's
+state f^n (pre-collision), ``moments()`` returns ``density_field(f^n)``
+and ``macroscopic_velocity(f^n, F)``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _device as dev
+from ._lib import (
+    BOUNDARY,
+    D3Q19_F32,
+    D3Q19_F64,
+    FACES,
+    FORCE_FIELD,
+    FORCE_NONE,
+    FORCE_UNIFORM,
+    STORAGE,
+    Config,
+    call,
+    load,
+)
+
+
+def _host_or_device(a, dtype_np):
+    """(pointer, dtype code, on_device, keepalive) for a NumPy array or a device tensor."""
+    if dev.is_tensor(a):
+        t = dev.torch()
+        if not a.is_cuda:
+            a = a.cpu().numpy()
+        else:
+            if a.dtype not in (t.float32, t.float64):
+                a = a.to(t.float64)
+            a = a.contiguous()
+            code = D3Q19_F64 if a.dtype == t.float64 else D3Q19_F32
+            return ctypes.c_void_p(a.data_ptr()), code, 1, a
+    arr = np.ascontiguousarray(a, dtype=dtype_np)
+    code = D3Q19_F64 if arr.dtype == np.float64 else D3Q19_F32
+    return ctypes.c_void_p(arr.ctypes.data), code, 0, arr
+
+
+class Lattice:
+    """A D3Q19 lattice resident on one GPU.
+
+    Parameters
+    ----------
+    shape : (Nx, Ny, Nz)
+    tau : BGK relaxation time (> 0.5)
+    storage : "f32" (default) | "f64" | "f16" | "bf16"  -- population storage;
+        16-bit formats store f_i - w_i, collision stays FP32.
+    boundary : "periodic" | "ywalls" (the reference's apply_bounce_back_y_walls)
+        | "halfway" (textbook half-way bounce-back on `wall_faces`, solids).
+    wall_velocity_top / wall_velocity_bottom : ywalls mode wall velocities.
+    wall_faces : iterable of "-x","+x","-y","+y","-z","+z" (halfway mode).
+    face_velocity : {face: (ux, uy, uz)} moving walls (halfway mode).
+    force : None, a uniform (Fx, Fy, Fz), or a (3, Nx, Ny, Nz) field (Guo).
+    """
+
+    def __init__(self, shape, tau, *, storage="f32", boundary="periodic",
+                 wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0),
+                 wall_faces=(), face_velocity=None, force=None, device=0, stream=None):
+        lib = load()
+        nx, ny, nz = (int(s) for s in shape)
+        self.shape = (nx, ny, nz)
+        self.ncell = nx * ny * nz
+        self.tau = float(tau)
+        self.storage = storage
+        self.boundary = boundary
+        self.device = int(device)
+        cfg = Config()
+        lib.d3q19_config_default(ctypes.byref(cfg))
+        cfg.nx, cfg.ny, cfg.nz = nx, ny, nz
+        cfg.tau = self.tau
+        try:
+            cfg.storage = STORAGE[storage]
+            cfg.boundary = BOUNDARY[boundary]
+        except KeyError as e:
+            raise ValueError(f"unknown storage/boundary {e}") from None
+        for a in range(3):
+            cfg.wall_velocity_top[a] = float(wall_velocity_top[a])
+            cfg.wall_velocity_bottom[a] = float(wall_velocity_bottom[a])
+        mask = 0
+        for face in wall_faces:
+            mask |= 1 << FACES[face]
+        cfg.wall_faces = mask
+        for face, vel in (face_velocity or {}).items():
+            for a in range(3):
+                cfg.face_velocity[FACES[face]][a] = float(vel[a])
+        self._force_field = None
+        if force is None:
+            cfg.force_mode = FORCE_NONE
+        elif np.ndim(force) == 1:
+            cfg.force_mode = FORCE_UNIFORM
+            for a in range(3):
+                cfg.force[a] = float(force[a])
+        else:
+            cfg.force_mode = FORCE_FIELD
+            self._force_field = force
+        cfg.device = self.device
+        self._cfg = cfg
+        self._stream = stream
+        h = ctypes.c_void_p()
+        call("d3q19_create", ctypes.byref(cfg), self._stream_arg(), ctypes.byref(h))
+        self._h = h
+        if self._force_field is not None:
+            self.set_force_field(self._force_field)
+
+    # ---- plumbing -------------------------------------------------------
+    def _stream_arg(self):
+        if self._stream is not None:
+            return ctypes.c_void_p(int(self._stream))
+        return dev.stream_ptr(self.device)
+
+    def _sync_stream(self):
+        call("d3q19_set_stream", self._h, self._stream_arg())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().d3q19_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- state ----------------------------------------------------------
+    def set_equilibrium(self, rho=None, u=None):
+        """State <- feq_field(rho, u) (equilibrium.py:71-98); rho None means 1, u None means 0."""
+        self._sync_stream()
+        grid = self.shape
+        if u is None:
+            u = np.zeros((3,) + grid, np.float64)
+        if tuple(u.shape) != (3,) + grid:
+            raise ValueError(f"u must have shape {(3,) + grid}")
+        pu, code, on_dev, keep_u = _host_or_device(u, np.float64)
+        prho = ctypes.c_void_p(None)
+        keep_r = None
+        if rho is not None:
+            if tuple(rho.shape) != grid:
+                raise ValueError(f"rho must have shape {grid}")
+            if dev.is_tensor(keep_u):
+                keep_r = rho.to(dtype=keep_u.dtype, device=keep_u.device).contiguous()
+                prho = ctypes.c_void_p(keep_r.data_ptr())
+            else:
+                keep_r = np.ascontiguousarray(rho, dtype=keep_u.dtype)
+                prho = ctypes.c_void_p(keep_r.ctypes.data)
+        call("d3q19_set_equilibrium", self._h, prho, pu, code, on_dev)
+        return self
+
+    def set_populations(self, f):
+        """State <- f (pre-collision populations, shape (19, Nx, Ny, Nz))."""
+        if tuple(f.shape) != (19,) + self.shape:
+            raise ValueError(f"populations must have shape {(19,) + self.shape}")
+        self._sync_stream()
+        p, code, on_dev, keep = _host_or_device(f, np.float64)
+        call("d3q19_set_populations", self._h, p, code, on_dev)
+        return self
+
+    def set_force_field(self, force):
+        """Per-cell Guo body force (3, Nx, Ny, Nz) used from the next collision on."""
+        if tuple(force.shape) != (3,) + self.shape:
+            raise ValueError(f"force must have shape {(3,) + self.shape}")
+        if self._cfg.force_mode != FORCE_FIELD:
+            raise ValueError("lattice was not created with a force field")
+        self._sync_stream()
+        p, code, on_dev, keep = _host_or_device(force, np.float64)
+        call("d3q19_set_force_field", self._h, p, code, on_dev)
+        self._force_keep = keep
+        return self
+
+    def set_solid(self, solid):
+        """Solid cells (Nx, Ny, Nz) bool/uint8 for the half-way mode; None clears."""
+        self._sync_stream()
+        if solid is None:
+            call("d3q19_set_solid", self._h, ctypes.c_void_p(None), 0)
+            return self
+        if dev.is_tensor(solid):
+            t = dev.torch()
+            s = solid.to(dtype=t.uint8, device=f"cuda:{self.device}").contiguous()
+            call("d3q19_set_solid", self._h, ctypes.c_void_p(s.data_ptr()), 1)
+        else:
+            s = np.ascontiguousarray(solid, dtype=np.uint8)
+            call("d3q19_set_solid", self._h, ctypes.c_void_p(s.ctypes.data), 0)
+        return self
+
+    def set_brinkman(self, s):
+        """Brinkman solid fraction in [0, 1] (Nx, Ny, Nz); None clears."""
+        self._sync_stream()
+        if s is None:
+            call("d3q19_set_brinkman", self._h, ctypes.c_void_p(None), D3Q19_F64, 0)
+            return self
+        p, code, on_dev, keep = _host_or_device(s, np.float64)
+        call("d3q19_set_brinkman", self._h, p, code, on_dev)
+        return self
+
+    # ---- stepping -------------------------------------------------------
+    def step(self, n: int = 1):
+        self._sync_stream()
+        call("d3q19_step", self._h, int(n))
+        return self
+
+    def step_moments(self, n: int, rho_out=None, u_out=None):
+        """Advance n steps; the last pass writes rho/u into FP32 device tensors."""
+        self._sync_stream()
+        call("d3q19_step_moments", self._h, int(n), dev.ptr(rho_out), dev.ptr(u_out))
+        return self
+
+    @property
+    def step_count(self) -> int:
+        return int(load().d3q19_step_count(self._h))
+
+    @property
+    def device_bytes(self) -> int:
+        return int(load().d3q19_device_bytes(self._h))
+
+    def synchronize(self):
+        call("d3q19_synchronize", self._h)
+
+    # ---- read-back ------------------------------------------------------
+    def populations(self, dtype=np.float64):
+        """f^n, the reference's pre-collision state, as a NumPy array."""
+        self._sync_stream()
+        out = np.empty((19,) + self.shape, dtype=dtype)
+        code = D3Q19_F64 if out.dtype == np.float64 else D3Q19_F32
+        call("d3q19_read_populations", self._h, ctypes.c_void_p(out.ctypes.data), code, 0)
+        return out
+
+    def populations_into(self, out):
+        """Write f^n into a (19, Nx, Ny, Nz) float32/float64 device tensor."""
+        self._sync_stream()
+        t = dev.torch()
+        code = D3Q19_F64 if out.dtype == t.float64 else D3Q19_F32
+        call("d3q19_read_populations", self._h, dev.ptr(out), code, 1)
+        return out
+
+    def moments(self, dtype=np.float64):
+        """(rho, u) of f^n: density_field and macroscopic_velocity(f, F)."""
+        self._sync_stream()
+        rho = np.empty(self.shape, dtype=dtype)
+        u = np.empty((3,) + self.shape, dtype=dtype)
+        code = D3Q19_F64 if rho.dtype == np.float64 else D3Q19_F32
+        call("d3q19_read_moments", self._h, ctypes.c_void_p(rho.ctypes.data),
+             ctypes.c_void_p(u.ctypes.data), code, 0)
+        return rho, u
+
+    def moments_into(self, rho=None, u=None):
+        self._sync_stream()
+        t = dev.torch()
+        ref = rho if rho is not None else u
+        code = D3Q19_F64 if ref.dtype == t.float64 else D3Q19_F32
+        call("d3q19_read_moments", self._h, dev.ptr(rho), dev.ptr(u), code, 1)
+        return rho, u
+
+    def diagnostics(self) -> dict:
+        """FP64 mass, momentum and kinetic energy of f^n."""
+        self._sync_stream()
+        out = (ctypes.c_double * 5)()
+        call("d3q19_diagnostics", self._h, out)
+        return {"mass": out[0], "momentum": (out[1], out[2], out[3]), "kinetic_energy": out[4]}
+
+    def enstrophy(self) -> float:
+        self._sync_stream()
+        out = ctypes.c_double()
+        call("d3q19_enstrophy", self._h, ctypes.byref(out))
+        return out.value

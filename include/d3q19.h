@@ -1,0 +1,146 @@
+/*
+ * d3q19.h -- C ABI of the MI355X D3Q19 lattice-Boltzmann engine (libd3q19.so).
+ *
+ * Drop-in for the step/read-back path of the reference package
+ * packages/lattice-boltzmann-d3q19 (paths below are relative to the
+ * reference root; NP = packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19).
+ *
+ * Conventions
+ *  - Populations cross the ABI as (19, Nx, Ny, Nz) arrays, C order (z fastest),
+ *    direction-major, in the reference's lexicographic velocity order
+ *    (NP/reference/constants.py:39-59).  Vector fields (velocity, force) are
+ *    (3, Nx, Ny, Nz); scalar fields (Nx, Ny, Nz).
+ *  - dtype codes: D3Q19_F64, D3Q19_F32 for arrays crossing the ABI; the
+ *    lattice storage type may also be D3Q19_F16 / D3Q19_BF16 (populations
+ *    kept on the GPU as f_i - w_i, collision in FP32).
+ *  - on_device != 0: the pointer is device memory on the lattice's device;
+ *    otherwise host memory (the library stages it).
+ *  - stream: a hipStream_t passed as void*; NULL selects the library's own
+ *    stream.  Calls are stream-ordered; read_* and diagnostics synchronise.
+ *  - Every function returns a d3q19_status; d3q19_last_error() gives the
+ *    message of the last failure on the calling thread.  No C++ exception
+ *    crosses the ABI.  A handle must not be used by two threads at once.
+ */
+#ifndef D3Q19_H
+#define D3Q19_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum d3q19_status {
+    D3Q19_OK = 0,
+    D3Q19_EINVAL = 1,  /* invalid argument or configuration */
+    D3Q19_ENOMEM = 2,  /* device allocation failed */
+    D3Q19_EHIP = 3,    /* HIP runtime error */
+    D3Q19_ESTATE = 4   /* call not valid in the lattice's current state */
+} d3q19_status;
+
+enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3 };
+
+enum {
+    D3Q19_BC_PERIODIC = 0, /* periodic on all six faces (NP/reference/bgk.py:115-126) */
+    D3Q19_BC_YWALLS = 1,   /* periodic, then apply_bounce_back_y_walls (NP/reference/bgk.py:145-216) */
+    D3Q19_BC_HALFWAY = 2   /* textbook half-way bounce-back on selected faces + solid cells */
+};
+
+enum { D3Q19_FORCE_NONE = 0, D3Q19_FORCE_UNIFORM = 1, D3Q19_FORCE_FIELD = 2 };
+
+/* Face indices for wall_faces / face_velocity (HALFWAY mode). */
+enum { D3Q19_FACE_XM = 0, D3Q19_FACE_XP, D3Q19_FACE_YM, D3Q19_FACE_YP, D3Q19_FACE_ZM, D3Q19_FACE_ZP };
+
+typedef struct d3q19_config {
+    int32_t nx, ny, nz;
+    double tau;                      /* BGK relaxation time, > 0.5 */
+    int32_t storage;                 /* D3Q19_F32 (default) | F64 | F16 | BF16 */
+    int32_t boundary;                /* D3Q19_BC_* */
+    /* D3Q19_BC_YWALLS: the wall_velocity_top / wall_velocity_bottom
+       arguments of apply_bounce_back_y_walls (NP/reference/bgk.py:145-149) */
+    double wall_velocity_top[3];
+    double wall_velocity_bottom[3];
+    /* D3Q19_BC_HALFWAY: bit k set = face k is a wall (else periodic) */
+    uint32_t wall_faces;
+    double face_velocity[6][3];
+    int32_t force_mode;              /* D3Q19_FORCE_*; FIELD uses d3q19_set_force_field */
+    double force[3];                 /* D3Q19_FORCE_UNIFORM body force (Guo) */
+    int32_t device;                  /* HIP device ordinal */
+} d3q19_config;
+
+typedef struct d3q19_lattice d3q19_lattice;
+
+/* ---- library ---------------------------------------------------------- */
+const char* d3q19_last_error(void);
+const char* d3q19_version(void);
+/* Fill cfg with: 0 extents, tau 0.6, F32 storage, periodic, no force, device 0. */
+void d3q19_config_default(d3q19_config* cfg);
+
+/* ---- stateful lattice (the step / read-back path) ---------------------
+ * Replaces the runner loop of NP/sim.py:312-326 / :350-364:
+ *   f = bgk_step(f, tau, force); f = apply_bounce_back_y_walls(f, top, bottom)
+ * with one fused stream-collide kernel per step. */
+int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out);
+int d3q19_destroy(d3q19_lattice* lat);
+int d3q19_set_stream(d3q19_lattice* lat, void* stream);
+/* Set the state to populations f (pre-collision, the reference's f^n). */
+int d3q19_set_populations(d3q19_lattice* lat, const void* f, int32_t dtype, int32_t on_device);
+/* Set the state to feq_field(rho, u) (NP/reference/equilibrium.py:71-98,
+   NP/sim.py:281-285).  rho may be NULL for rho = 1. */
+int d3q19_set_equilibrium(d3q19_lattice* lat, const void* rho, const void* u, int32_t dtype,
+                          int32_t on_device);
+/* Per-cell body force (3,Nx,Ny,Nz) for D3Q19_FORCE_FIELD (bgk.py:45-49 force_lattice). */
+int d3q19_set_force_field(d3q19_lattice* lat, const void* force, int32_t dtype, int32_t on_device);
+/* HALFWAY mode: solid cells (Nx,Ny,Nz) uint8, nonzero = solid; NULL clears. */
+int d3q19_set_solid(d3q19_lattice* lat, const uint8_t* solid, int32_t on_device);
+/* Brinkman solid fraction s in [0,1] (Nx,Ny,Nz): body force F - s * rho u. NULL clears. */
+int d3q19_set_brinkman(d3q19_lattice* lat, const void* s, int32_t dtype, int32_t on_device);
+/* Advance n steps. */
+int d3q19_step(d3q19_lattice* lat, int32_t n);
+/* Advance n >= 1 steps and write rho (Nx,Ny,Nz) and u (3,Nx,Ny,Nz) of the new
+   state from the last step's own pass (device FP32 pointers, either may be NULL). */
+int d3q19_step_moments(d3q19_lattice* lat, int32_t n, float* rho_dev, float* u_dev);
+/* Pre-collision populations f^n, as the reference holds them after step n. */
+int d3q19_read_populations(d3q19_lattice* lat, void* f, int32_t dtype, int32_t on_device);
+/* density_field(f^n) and macroscopic_velocity(f^n, force) (bgk.py:129-142).
+   Either output may be NULL. */
+int d3q19_read_moments(d3q19_lattice* lat, void* rho, void* u, int32_t dtype, int32_t on_device);
+/* FP64 reductions over fluid cells of f^n: out[0] mass, out[1..3] momentum
+   sum(c f), out[4] kinetic energy 0.5 sum(rho |u|^2) with the read-back u. */
+int d3q19_diagnostics(d3q19_lattice* lat, double out[5]);
+/* FP64 enstrophy 0.5 sum |curl u|^2 (periodic central differences) of the read-back u. */
+int d3q19_enstrophy(d3q19_lattice* lat, double* out);
+int d3q19_synchronize(d3q19_lattice* lat);
+int64_t d3q19_step_count(const d3q19_lattice* lat);
+size_t d3q19_device_bytes(const d3q19_lattice* lat);
+
+/* ---- stateless drop-ins (NP/reference/__init__.py:43-71) ---------------
+ * Device pointers, dtype D3Q19_F64 or D3Q19_F32, (19,Nx,Ny,Nz) layout,
+ * stream-ordered on `stream`.  FP64 results follow the reference's operation
+ * order (no FMA contraction). */
+/* bgk_step (NP/reference/bgk.py:45-112); force may be NULL. */
+int d3q19_bgk_step(const void* f, void* out, const void* force, int32_t nx, int32_t ny, int32_t nz,
+                   double tau, int32_t dtype, void* stream);
+/* stream (NP/reference/bgk.py:115-126). */
+int d3q19_stream(const void* f, void* out, int32_t nx, int32_t ny, int32_t nz, int32_t dtype,
+                 void* stream);
+/* apply_bounce_back_y_walls (NP/reference/bgk.py:145-216). */
+int d3q19_bounce_back_y_walls(const void* f, void* out, const double wall_velocity_top[3],
+                              const double wall_velocity_bottom[3], int32_t nx, int32_t ny,
+                              int32_t nz, int32_t dtype, void* stream);
+/* feq_field (NP/reference/equilibrium.py:71-98). */
+int d3q19_feq_field(const void* rho, const void* u, void* out, int64_t ncell, int32_t dtype,
+                    void* stream);
+/* density_field (NP/reference/equilibrium.py:101-103). */
+int d3q19_density_field(const void* f, void* rho, int64_t ncell, int32_t dtype, void* stream);
+/* momentum_field (NP/reference/equilibrium.py:106-113). */
+int d3q19_momentum_field(const void* f, void* mom, int64_t ncell, int32_t dtype, void* stream);
+/* macroscopic_velocity (NP/reference/bgk.py:129-142); force may be NULL. */
+int d3q19_macroscopic_velocity(const void* f, const void* force, void* u, int64_t ncell,
+                               int32_t dtype, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* D3Q19_H */

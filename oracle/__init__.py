@@ -1,0 +1,236 @@
+"""FP64 CPU oracle for the D3Q19 BGK step -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this package, and only as the checker.  The product
+(``bit-physics_amd/``) never imports it.
+
+It wraps ``oracle/d3q19_oracle.c`` (built to ``oracle/_build/``) with NumPy
+in/out functions named after the reference API of
+``packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/__init__.py:43-71``.
+Every array is float64, C-contiguous, direction-major ``(19, Nx, Ny, Nz)``.
+
+Pinning (what checks each function; see tests/test_oracle_*.py):
+  feq, feq_field          golden table d3q19-equilibrium.json (abs 1e-15),
+                          sum(W)==1, feq(1,0)==W, moment identities (PBT)
+  density/momentum_field  closed-form moments of feq, brute-force sums
+  stream                  np.roll (library routine) on random fields
+  bgk_step                rest-state fixed point, exact mass/momentum
+                          conservation, MMS ladder 4.224e-3/3.033e-3/1.543e-4
+  bounce_back_y_walls     reference golden vectors (tests/golden), the
+                          published Poiseuille/Couette maxima
+  run_halfway             Poiseuille parabola at the exact-wall tau,
+                          Couette linear profile, mass conservation
+  diagnostics             brute-force NumPy sums
+  inverse_grad            central finite differences of inverse_loss
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "_build" / "libd3q19_oracle.so"
+_lib = None
+
+Q = 19
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (gcc, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-B", "-C", str(_HERE)], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        build()
+    lib = ctypes.CDLL(str(_LIB_PATH))
+    P = ctypes.c_void_p
+    i32, i64, f64, u32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32
+    sigs = {
+        "orc_constants": (None, [P, P, P, P]),
+        "orc_feq_point": (None, [f64, f64, f64, f64, P]),
+        "orc_feq_field": (None, [i64, P, P, P]),
+        "orc_density_field": (None, [i64, P, P]),
+        "orc_momentum_field": (None, [i64, P, P]),
+        "orc_macroscopic_velocity": (None, [i64, P, P, P]),
+        "orc_collide": (None, [i64, P, P, f64, P]),
+        "orc_stream": (None, [i32, i32, i32, P, P]),
+        "orc_bgk_step": (None, [i32, i32, i32, P, P, f64, P, P]),
+        "orc_bounce_back_y_walls": (None, [i32, i32, i32, P, P, P, P]),
+        "orc_run_reference": (None, [i32, i32, i32, P, f64, P, i32, P, P, i32, P]),
+        "orc_stream_halfway": (None, [i32, i32, i32, P, P, u32, P, P]),
+        "orc_run_halfway": (None, [i32, i32, i32, P, f64, P, P, P, u32, P, P, i32, P]),
+        "orc_diagnostics": (None, [i64, P, P, P, P]),
+        "orc_inverse_loss": (f64, [i32, i32, i32, P, P, P, f64, i32]),
+        "orc_inverse_grad": (f64, [i32, i32, i32, P, P, P, f64, i32, P, P]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def constants():
+    lib = _load()
+    c = np.zeros((19, 3), np.int32)
+    w = np.zeros(19, np.float64)
+    opp = np.zeros(19, np.int32)
+    cs2 = ctypes.c_double()
+    lib.orc_constants(_p(c), _p(w), _p(opp), ctypes.byref(cs2))
+    return c, w, opp, cs2.value
+
+
+def feq(rho: float, u) -> list[float]:
+    out = np.zeros(19, np.float64)
+    _load().orc_feq_point(float(rho), float(u[0]), float(u[1]), float(u[2]), _p(out))
+    return out.tolist()
+
+
+def feq_field(rho, u):
+    rho = _f64(rho)
+    u = _f64(u)
+    out = np.empty((19,) + rho.shape, np.float64)
+    _load().orc_feq_field(rho.size, _p(rho), _p(u), _p(out))
+    return out
+
+
+def density_field(f):
+    f = _f64(f)
+    out = np.empty(f.shape[1:], np.float64)
+    _load().orc_density_field(out.size, _p(f), _p(out))
+    return out
+
+
+def momentum_field(f):
+    f = _f64(f)
+    out = np.empty((3,) + f.shape[1:], np.float64)
+    _load().orc_momentum_field(out[0].size, _p(f), _p(out))
+    return out
+
+
+def macroscopic_velocity(f, force_lattice=None):
+    f = _f64(f)
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3,) + f.shape[1:]))
+    out = np.empty((3,) + f.shape[1:], np.float64)
+    _load().orc_macroscopic_velocity(out[0].size, _p(f), _p(F), _p(out))
+    return out
+
+
+def collide(f, tau, force_lattice=None):
+    f = _f64(f)
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3,) + f.shape[1:]))
+    out = np.empty_like(f)
+    _load().orc_collide(f[0].size, _p(f), _p(out), float(tau), _p(F))
+    return out
+
+
+def stream(f_post):
+    f_post = _f64(f_post)
+    out = np.empty_like(f_post)
+    nx, ny, nz = f_post.shape[1:]
+    _load().orc_stream(nx, ny, nz, _p(f_post), _p(out))
+    return out
+
+
+def bgk_step(f, tau, force_lattice=None):
+    f = _f64(f)
+    nx, ny, nz = f.shape[1:]
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3, nx, ny, nz)))
+    out = np.empty_like(f)
+    scratch = np.empty_like(f)
+    _load().orc_bgk_step(nx, ny, nz, _p(f), _p(out), float(tau), _p(F), _p(scratch))
+    return out
+
+
+def apply_bounce_back_y_walls(f, wall_velocity_top=(0.0, 0.0, 0.0),
+                              wall_velocity_bottom=(0.0, 0.0, 0.0)):
+    f = _f64(f)
+    nx, ny, nz = f.shape[1:]
+    top = _f64(wall_velocity_top)
+    bot = _f64(wall_velocity_bottom)
+    out = np.empty_like(f)
+    _load().orc_bounce_back_y_walls(nx, ny, nz, _p(f), _p(out), _p(top), _p(bot))
+    return out
+
+
+def run_reference(f, tau, steps, force_lattice=None, ywalls=False,
+                  wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0)):
+    """NP/sim.py:312-326 loop: bgk_step then (optionally) the y-wall bounce-back."""
+    f = _f64(f).copy()
+    nx, ny, nz = f.shape[1:]
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3, nx, ny, nz)))
+    top = _f64(wall_velocity_top)
+    bot = _f64(wall_velocity_bottom)
+    work = np.empty((2,) + f.shape, np.float64)
+    _load().orc_run_reference(nx, ny, nz, _p(f), float(tau), _p(F), int(bool(ywalls)),
+                              _p(top), _p(bot), int(steps), _p(work))
+    return f
+
+
+FACE_NAMES = ("-x", "+x", "-y", "+y", "-z", "+z")
+
+
+def run_halfway(f, tau, steps, wall_mask=0, face_velocity=None, solid=None,
+                force_field=None, force_uniform=None, brinkman=None):
+    """Half-way bounce-back engine mode (no reference counterpart)."""
+    f = _f64(f).copy()
+    nx, ny, nz = f.shape[1:]
+    fu = np.zeros((6, 3)) if face_velocity is None else face_velocity
+    fu = _f64(fu)
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8)
+    F = None if force_field is None else _f64(force_field)
+    Fu = None if force_uniform is None else _f64(force_uniform)
+    B = None if brinkman is None else _f64(brinkman)
+    work = np.empty_like(f)
+    _load().orc_run_halfway(nx, ny, nz, _p(f), float(tau), _p(F), _p(Fu), _p(B),
+                            int(wall_mask), _p(fu), _p(sol), int(steps), _p(work))
+    return f
+
+
+def diagnostics(f, force_uniform=None, solid=None):
+    """[mass, px, py, pz, kinetic energy] in FP64 (sequential sums)."""
+    f = _f64(f)
+    out = np.zeros(5, np.float64)
+    Fu = None if force_uniform is None else _f64(force_uniform)
+    sol = None if solid is None else np.ascontiguousarray(solid, dtype=np.uint8)
+    _load().orc_diagnostics(f[0].size, _p(f), _p(Fu), _p(sol), _p(out))
+    return out
+
+
+def inverse_loss(u0, F, u_tgt, tau, T):
+    u0, F, u_tgt = _f64(u0), _f64(F), _f64(u_tgt)
+    nx, ny, nz = u0.shape[1:]
+    return _load().orc_inverse_loss(nx, ny, nz, _p(u0), _p(F), _p(u_tgt), float(tau), int(T))
+
+
+def inverse_grad(u0, F, u_tgt, tau, T):
+    u0, F, u_tgt = _f64(u0), _f64(F), _f64(u_tgt)
+    nx, ny, nz = u0.shape[1:]
+    g_u0 = np.zeros_like(u0)
+    g_F = np.zeros_like(F)
+    L = _load().orc_inverse_grad(nx, ny, nz, _p(u0), _p(F), _p(u_tgt), float(tau), int(T),
+                                 _p(g_u0), _p(g_F))
+    return L, g_u0, g_F
+
+
+def num_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))

@@ -1,0 +1,97 @@
+"""The C ABI library loads and exports every symbol include/*.h declares;
+the Python mirror of d3q19_config has the C layout (CPU, no compute calls)."""
+
+from __future__ import annotations
+
+import ctypes
+import re
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADERS = sorted((ROOT / "include").glob("*.h"))
+
+
+def declared_functions():
+    names = []
+    for h in HEADERS:
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"^\s*[A-Za-z_][\w\s\*]*?\b(d3q19_\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_headers_declare_functions():
+    names = declared_functions()
+    assert "d3q19_step" in names and "d3q19_bgk_step" in names
+    assert len(names) >= 27
+
+
+def test_library_exports_every_declared_symbol():
+    from lattice_boltzmann_d3q19_amd._lib import LIB_PATH
+
+    if not LIB_PATH.exists():
+        pytest.fail(f"{LIB_PATH} not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, f"not exported: {missing}"
+    # the version string needs no GPU
+    lib.d3q19_version.restype = ctypes.c_char_p
+    assert b"gfx950" in lib.d3q19_version()
+
+
+def test_python_binding_covers_header():
+    from lattice_boltzmann_d3q19_amd._lib import EXPORTED
+
+    assert set(declared_functions()) <= set(EXPORTED)
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="gcc needed to read the C layout")
+def test_config_struct_layout_matches_c(tmp_path):
+    from lattice_boltzmann_d3q19_amd._lib import Config
+
+    fields = [f[0] for f in Config._fields_]
+    src = tmp_path / "layout.c"
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{ROOT / "include" / "d3q19.h"}"',
+             "int main(void) {", 'printf("%zu\\n", sizeof(d3q19_config));']
+    lines += [f'printf("%zu\\n", offsetof(d3q19_config, {f}));' for f in fields]
+    lines += ["return 0; }"]
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-o", str(exe), str(src)], check=True)
+    out = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert out[0] == ctypes.sizeof(Config)
+    for f, off in zip(fields, out[1:]):
+        assert getattr(Config, f).offset == off, f
+
+
+def test_config_default_needs_no_gpu():
+    from lattice_boltzmann_d3q19_amd._lib import Config, load
+
+    cfg = Config()
+    load().d3q19_config_default(ctypes.byref(cfg))
+    assert cfg.tau == 0.6 and cfg.storage == 1 and cfg.boundary == 0
+
+
+def test_create_rejects_bad_config_without_gpu():
+    """Argument validation runs before any HIP call."""
+    from lattice_boltzmann_d3q19_amd._lib import Config, load
+
+    lib = load()
+    cfg = Config()
+    lib.d3q19_config_default(ctypes.byref(cfg))
+    cfg.nx = cfg.ny = cfg.nz = 4
+    h = ctypes.c_void_p()
+    cfg.tau = 0.5
+    assert lib.d3q19_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1  # EINVAL
+    assert b"tau" in lib.d3q19_last_error()
+    cfg.tau = 0.6
+    cfg.storage = 9
+    assert lib.d3q19_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1
+    cfg.storage = 1
+    cfg.nx = 0
+    assert lib.d3q19_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1
+    assert not h.value

@@ -1,0 +1,240 @@
+"""GPU parity of the stateful fused step (Lattice over the C ABI) against the
+FP64 oracle and the reference's known answers.
+
+Tolerances: FP64 storage 1e-12 field-scaled; FP32 storage 1e-6 field-scaled
+(populations); moments as stated per test.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import GOLDEN, field_scaled_err, print_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "f32": 1e-6}
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import lattice_boltzmann_d3q19_amd as m
+
+    return m
+
+
+def seeded(shape, seed=42, amp=0.01):
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + amp * rng.standard_normal(shape)
+    u = amp * rng.standard_normal((3,) + shape)
+    f = oracle.feq_field(rho, u) * (1.0 + 0.02 * rng.standard_normal((19,) + shape))
+    F = 1e-4 * rng.standard_normal((3,) + shape)
+    return f, F
+
+
+def rest(shape):
+    return oracle.feq_field(np.ones(shape), np.zeros((3,) + shape))
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(7, 5, 9), (16, 8, 3), (1, 1, 1), (4, 3, 1)])
+def test_periodic_guo_field_matches_oracle(amd, storage, shape):
+    print_case("lattice-periodic-guo", storage=storage, shape=shape)
+    f0, F = seeded(shape)
+    steps = 20
+    lat = amd.Lattice(shape, 0.6, storage=storage, force=F)
+    lat.set_populations(f0).step(steps)
+    ref = oracle.run_reference(f0, 0.6, steps, force_lattice=F)
+    assert field_scaled_err(lat.populations(), ref) < TOL[storage]
+    rho, u = lat.moments()
+    assert field_scaled_err(rho, oracle.density_field(ref)) < TOL[storage]
+    assert field_scaled_err(u, oracle.macroscopic_velocity(ref, F)) < 100 * TOL[storage]
+    assert lat.step_count == steps
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_force_free_and_uniform_force(amd, storage):
+    print_case("lattice-uniform-force", storage=storage)
+    shape = (6, 10, 5)
+    f0, _ = seeded(shape, seed=1)
+    lat = amd.Lattice(shape, 0.8, storage=storage)
+    lat.set_populations(f0).step(15)
+    assert field_scaled_err(lat.populations(), oracle.run_reference(f0, 0.8, 15)) < TOL[storage]
+    Fu = (2e-5, -1e-5, 3e-5)
+    lat = amd.Lattice(shape, 0.8, storage=storage, force=Fu)
+    lat.set_populations(f0).step(15)
+    Ff = np.broadcast_to(np.asarray(Fu)[:, None, None, None], (3,) + shape)
+    assert field_scaled_err(lat.populations(), oracle.run_reference(f0, 0.8, 15, force_lattice=Ff)) < TOL[storage]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_ywalls_moving_matches_reference_golden(amd, storage):
+    print_case("lattice-ywalls-golden", storage=storage)
+    d = np.load(GOLDEN / "ywalls_moving.npz", allow_pickle=False)
+    shape = d["f0"].shape[1:]
+    lat = amd.Lattice(shape, float(d["tau"]), storage=storage, boundary="ywalls",
+                      wall_velocity_top=d["wall_top"], wall_velocity_bottom=d["wall_bottom"],
+                      force=d["force"])
+    lat.set_populations(d["f0"])
+    for k in range(d["composite"].shape[0]):
+        lat.step(1)
+        assert field_scaled_err(lat.populations(), d["composite"][k]) < TOL[storage], k
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_ywalls_long_run_matches_oracle(amd, storage):
+    print_case("lattice-ywalls-long", storage=storage)
+    shape = (10, 7, 4)
+    f0, F = seeded(shape, seed=3)
+    top, bot = (0.04, 0.0, -0.01), (0.0, 0.0, 0.02)
+    lat = amd.Lattice(shape, 0.65, storage=storage, boundary="ywalls",
+                      wall_velocity_top=top, wall_velocity_bottom=bot, force=F)
+    lat.set_populations(f0).step(200)
+    ref = oracle.run_reference(f0, 0.65, 200, force_lattice=F, ywalls=True,
+                               wall_velocity_top=top, wall_velocity_bottom=bot)
+    assert field_scaled_err(lat.populations(), ref) < TOL[storage]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_poiseuille_known_answer_on_gpu(amd, storage):
+    """equivalence.md:240-241: max|u| 8.6508320e-03, max rho 1.0000000 at step 1000."""
+    print_case("lattice-poiseuille", storage=storage)
+    shape = (64, 32, 3)
+    lat = amd.Lattice(shape, 0.7, storage=storage, boundary="ywalls", force=(1e-5, 0.0, 0.0))
+    lat.set_equilibrium().step(1000)
+    rho, u = lat.moments()
+    umax = float(np.max(np.sqrt((u * u).sum(0))))
+    if storage == "f64":
+        assert f"{umax:.7e}" == "8.6508320e-03"
+        assert f"{rho.max():.7e}" == "1.0000000e+00"
+    else:
+        assert abs(umax - 8.6508320e-03) / 8.6508320e-03 < 1e-5
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_couette_known_answer_on_gpu(amd, storage):
+    """equivalence.md:243: max|u| 5.0000000e-02 at step 500."""
+    print_case("lattice-couette", storage=storage)
+    shape = (32, 16, 3)
+    lat = amd.Lattice(shape, 0.7, storage=storage, boundary="ywalls", wall_velocity_top=(0.05, 0.0, 0.0))
+    lat.set_equilibrium().step(500)
+    _, u = lat.moments()
+    umax = float(np.max(np.sqrt((u * u).sum(0))))
+    if storage == "f64":
+        assert f"{umax:.7e}" == "5.0000000e-02"
+    else:
+        assert abs(umax - 0.05) / 0.05 < 1e-6
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_halfway_cavity_matches_oracle(amd, storage):
+    print_case("lattice-cavity-small", storage=storage)
+    shape = (12, 10, 14)
+    fv = np.zeros((6, 3))
+    fv[5] = (0.05, 0.0, 0.0)
+    lat = amd.Lattice(shape, 0.6, storage=storage, boundary="halfway",
+                      wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                      face_velocity={"+z": (0.05, 0.0, 0.0)})
+    lat.set_equilibrium().step(150)
+    ref = oracle.run_halfway(rest(shape), 0.6, 150, wall_mask=63, face_velocity=fv)
+    assert field_scaled_err(lat.populations(), ref) < TOL[storage]
+    rho, u = lat.moments()
+    assert field_scaled_err(u, oracle.macroscopic_velocity(ref)) < 100 * TOL[storage]
+
+
+def test_cavity_64_cubed_1000_steps_fp32(amd):
+    """Lid-driven cavity at its full size: 64^3, tau 0.6, lid 0.05 +x on the top z face."""
+    print_case("lattice-cavity-64", storage="f32")
+    shape = (64, 64, 64)
+    fv = np.zeros((6, 3))
+    fv[5] = (0.05, 0.0, 0.0)
+    lat = amd.Lattice(shape, 0.6, storage="f32", boundary="halfway",
+                      wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                      face_velocity={"+z": (0.05, 0.0, 0.0)})
+    lat.set_equilibrium().step(1000)
+    ref = oracle.run_halfway(rest(shape), 0.6, 1000, wall_mask=63, face_velocity=fv)
+    assert field_scaled_err(lat.populations(), ref) < 1e-6
+    rho, u = lat.moments()
+    assert field_scaled_err(u, oracle.macroscopic_velocity(ref)) < 1e-4
+    d = lat.diagnostics()
+    assert abs(d["mass"] - np.prod(shape)) / np.prod(shape) < 1e-6
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_halfway_solids_force_brinkman(amd, storage):
+    print_case("lattice-solids", storage=storage)
+    shape = (10, 9, 8)
+    rng = np.random.default_rng(11)
+    solid = rng.random(shape) < 0.1
+    brink = np.where(rng.random(shape) < 0.3, rng.random(shape), 0.0)
+    fv = np.zeros((6, 3))
+    fv[3] = (0.02, 0.0, 0.01)
+    lat = amd.Lattice(shape, 0.7, storage=storage, boundary="halfway", wall_faces=("-y", "+y"),
+                      face_velocity={"+y": (0.02, 0.0, 0.01)}, force=(1e-5, 0.0, 0.0))
+    lat.set_solid(solid).set_brinkman(brink).set_equilibrium().step(60)
+    ref = oracle.run_halfway(rest(shape), 0.7, 60, wall_mask=12, face_velocity=fv, solid=solid,
+                             force_uniform=(1e-5, 0.0, 0.0), brinkman=brink)
+    got = lat.populations()
+    fluid = ~solid
+    assert field_scaled_err(got[:, fluid], ref[:, fluid]) < TOL[storage]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_fp64_diagnostics_match_oracle(amd, storage):
+    print_case("lattice-diagnostics", storage=storage)
+    shape = (12, 8, 10)
+    f0, _ = seeded(shape, seed=5)
+    lat = amd.Lattice(shape, 0.6, storage=storage, force=(1e-5, 0, 0))
+    lat.set_populations(f0).step(10)
+    Ff = np.broadcast_to(np.asarray([1e-5, 0, 0])[:, None, None, None], (3,) + shape)
+    ref = oracle.run_reference(f0, 0.6, 10, force_lattice=Ff)
+    want = oracle.diagnostics(ref, force_uniform=(1e-5, 0, 0))
+    d = lat.diagnostics()
+    rel = 1e-13 if storage == "f64" else 1e-7
+    assert d["mass"] == pytest.approx(want[0], rel=rel)
+    assert np.allclose(d["momentum"], want[1:4], rtol=0, atol=rel * np.prod(shape) * 0.01 + 1e-12)
+    assert d["kinetic_energy"] == pytest.approx(want[4], rel=1e-11 if storage == "f64" else 1e-5)
+
+
+def test_step_moments_equal_readback(amd):
+    import torch
+
+    print_case("lattice-step-moments")
+    shape = (16, 12, 20)
+    f0, F = seeded(shape, seed=8)
+    lat = amd.Lattice(shape, 0.6, storage="f32", force=F)
+    lat.set_populations(f0)
+    rho = torch.empty(shape, dtype=torch.float32, device="cuda")
+    u = torch.empty((3,) + shape, dtype=torch.float32, device="cuda")
+    lat.step_moments(5, rho, u)
+    r2, u2 = lat.moments(dtype=np.float32)
+    assert np.array_equal(rho.cpu().numpy(), r2)
+    assert np.allclose(u.cpu().numpy(), u2, rtol=0, atol=1e-9)
+
+
+def test_deterministic_bitwise(amd):
+    print_case("lattice-determinism")
+    shape = (20, 10, 12)
+    f0, F = seeded(shape, seed=2)
+    outs = []
+    for _ in range(2):
+        lat = amd.Lattice(shape, 0.6, storage="f32", force=F)
+        lat.set_populations(f0).step(30)
+        outs.append((lat.populations(np.float32), lat.diagnostics()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert outs[0][1] == outs[1][1]
+
+
+def test_state_errors(amd):
+    print_case("lattice-errors")
+    lat = amd.Lattice((4, 4, 4), 0.6)
+    with pytest.raises(amd.D3Q19Error, match="ESTATE"):
+        lat.step(1)
+    with pytest.raises(amd.D3Q19Error, match="EINVAL"):
+        amd.Lattice((4, 4, 4), 0.5)
+    with pytest.raises(amd.D3Q19Error, match="EINVAL"):
+        lat.set_solid(np.zeros((4, 4, 4), np.uint8))  # periodic lattice has no solids
+    with pytest.raises(ValueError):
+        lat.set_populations(np.zeros((19, 4, 4, 5)))
