@@ -11,25 +11,27 @@
 #include <string>
 
 #include "../../include/d3q19.h"
+#include "d3q19_host.hpp"
 #include "d3q19_kernels.hpp"
 
 using namespace d3q19;
 
-namespace {
+namespace d3q19 {
 
 thread_local std::string g_err;
 
-int fail(int code, const std::string& msg) {
+int set_error(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
 
-#define HIPCHK(expr)                                                                           \
-    do {                                                                                       \
-        hipError_t e_ = (expr);                                                                \
-        if (e_ != hipSuccess)                                                                  \
-            return fail(D3Q19_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));        \
-    } while (0)
+}  // namespace d3q19
+
+namespace {
+
+inline int fail(int code, const std::string& msg) { return set_error(code, msg); }
+
+#define HIPCHK(expr) D3Q19_HIPCHK(expr)
 
 inline uint32_t nblocks(uint64_t n) { return (uint32_t)((n + BLOCK - 1) / BLOCK); }
 
@@ -215,43 +217,9 @@ int launch_diag(d3q19_lattice* L) {
 bool is_ct_double(const d3q19_lattice* L) { return L->cfg.storage == D3Q19_F64; }
 size_t ct_size(const d3q19_lattice* L) { return is_ct_double(L) ? 8 : 4; }
 
-// Stage a host or device array of n elements of dtype `dt` into a device
-// buffer of the lattice's compute type (double or float).
-__global__ void k_convert_f32_to_f64(const float* a, double* b, size_t n) {
-    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) b[i] = a[i];
-}
-__global__ void k_convert_f64_to_f32(const double* a, float* b, size_t n) {
-    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
-}
-
 int stage_to_compute(d3q19_lattice* L, const void* src, int32_t dt, int32_t on_device, size_t n,
                      void* dst) {
-    if (dt != D3Q19_F64 && dt != D3Q19_F32) return fail(D3Q19_EINVAL, "dtype must be F64 or F32");
-    const size_t in_es = dt == D3Q19_F64 ? 8 : 4;
-    const bool want_double = is_ct_double(L);
-    const bool same = (dt == D3Q19_F64) == want_double;
-    if (same) {
-        HIPCHK(hipMemcpyAsync(dst, src, n * in_es,
-                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, L->stream));
-        if (!on_device) HIPCHK(hipStreamSynchronize(L->stream));
-        return D3Q19_OK;
-    }
-    DevBuf tmp;
-    const void* dsrc = src;
-    if (!on_device) {
-        if (tmp.alloc(n * in_es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
-        HIPCHK(hipMemcpyAsync(tmp.p, src, n * in_es, hipMemcpyHostToDevice, L->stream));
-        dsrc = tmp.p;
-    }
-    if (want_double)
-        k_convert_f32_to_f64<<<nblocks(n), BLOCK, 0, L->stream>>>((const float*)dsrc, (double*)dst, n);
-    else
-        k_convert_f64_to_f32<<<nblocks(n), BLOCK, 0, L->stream>>>((const double*)dsrc, (float*)dst, n);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return stage_in(src, dt, on_device, n, dst, is_ct_double(L), L->stream);
 }
 
 int check_lat(const d3q19_lattice* L) {
@@ -265,6 +233,82 @@ int check_dtype(int32_t dt) {
 }
 
 }  // namespace
+
+namespace d3q19 {
+
+static __global__ void k_convert_f32_to_f64(const float* a, double* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) b[i] = a[i];
+}
+static __global__ void k_convert_f64_to_f32(const double* a, float* b, size_t n) {
+    size_t i = (size_t)blockIdx.x * BLOCK + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+namespace {
+struct Tmp {
+    void* p = nullptr;
+    ~Tmp() {
+        if (p) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+int stage_in(const void* src, int dt, int on_device, size_t n, void* dst, bool dst_double, hipStream_t s) {
+    if (dt != D3Q19_F64 && dt != D3Q19_F32) return set_error(D3Q19_EINVAL, "dtype must be F64 or F32");
+    if (!src || !dst) return set_error(D3Q19_EINVAL, "null array");
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    if ((dt == D3Q19_F64) == dst_double) {
+        D3Q19_HIPCHK(hipMemcpyAsync(dst, src, n * es, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        if (!on_device) D3Q19_HIPCHK(hipStreamSynchronize(s));
+        return D3Q19_OK;
+    }
+    Tmp tmp;
+    const void* d = src;
+    if (!on_device) {
+        if (hipMalloc(&tmp.p, n * es) != hipSuccess) return set_error(D3Q19_ENOMEM, "staging allocation failed");
+        D3Q19_HIPCHK(hipMemcpyAsync(tmp.p, src, n * es, hipMemcpyHostToDevice, s));
+        d = tmp.p;
+    }
+    if (dst_double)
+        k_convert_f32_to_f64<<<grid_for(n, BLOCK), BLOCK, 0, s>>>((const float*)d, (double*)dst, n);
+    else
+        k_convert_f64_to_f32<<<grid_for(n, BLOCK), BLOCK, 0, s>>>((const double*)d, (float*)dst, n);
+    D3Q19_HIPCHK(hipGetLastError());
+    D3Q19_HIPCHK(hipStreamSynchronize(s));
+    return D3Q19_OK;
+}
+
+int stage_out(const void* src, bool src_double, size_t n, void* dst, int dt, int on_device, hipStream_t s) {
+    if (dt != D3Q19_F64 && dt != D3Q19_F32) return set_error(D3Q19_EINVAL, "dtype must be F64 or F32");
+    if (!src || !dst) return set_error(D3Q19_EINVAL, "null array");
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    const bool same = (dt == D3Q19_F64) == src_double;
+    Tmp tmp;
+    const void* from = src;
+    if (!same) {
+        void* conv = dst;
+        if (!on_device) {
+            if (hipMalloc(&tmp.p, n * es) != hipSuccess) return set_error(D3Q19_ENOMEM, "staging allocation failed");
+            conv = tmp.p;
+        }
+        if (src_double)
+            k_convert_f64_to_f32<<<grid_for(n, BLOCK), BLOCK, 0, s>>>((const double*)src, (float*)conv, n);
+        else
+            k_convert_f32_to_f64<<<grid_for(n, BLOCK), BLOCK, 0, s>>>((const float*)src, (double*)conv, n);
+        D3Q19_HIPCHK(hipGetLastError());
+        from = conv;
+        if (on_device) {
+            D3Q19_HIPCHK(hipStreamSynchronize(s));
+            return D3Q19_OK;
+        }
+    }
+    D3Q19_HIPCHK(hipMemcpyAsync(dst, from, n * es, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    D3Q19_HIPCHK(hipStreamSynchronize(s));
+    return D3Q19_OK;
+}
+
+}  // namespace d3q19
 
 extern "C" {
 

@@ -1,0 +1,114 @@
+"""GPU adjoint (d3q19_adjoint_* through the C ABI) against the FP64 CPU
+adjoint of the oracle and against central finite differences of the GPU
+forward map (no reference counterpart: the reference has no adjoint; its
+gradient-check pattern is common/common-warp/src/common_warp/autodiff/finite_diff.py:26-47).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import field_scaled_err, print_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    import torch
+
+    from lattice_boltzmann_d3q19_amd import autograd as ag
+
+    return torch, ag
+
+
+def problem(shape, seed=42, rms=0.02, famp=1e-4):
+    rng = np.random.default_rng(seed)
+    u0 = rms * rng.standard_normal((3,) + shape)
+    F = famp * rng.standard_normal((3,) + shape)
+    ut = rms * rng.standard_normal((3,) + shape)
+    return u0, F, ut
+
+
+def gpu_grad(torch, ag, shape, tau, T, K, storage, u0, F, ut):
+    solver = ag.DifferentiableLBM(shape, tau, T, checkpoint_every=K, storage=storage)
+    u0t = torch.tensor(u0, device="cuda", requires_grad=True)
+    Ft = torch.tensor(F, device="cuda", requires_grad=True)
+    utt = torch.tensor(ut, device="cuda")
+    loss = ag.velocity_matching_loss(solver(u0t, Ft), utt)
+    loss.backward()
+    return float(loss), u0t.grad.cpu().numpy(), Ft.grad.cpu().numpy(), solver
+
+
+@pytest.mark.parametrize("T,K", [(7, 3), (6, 6), (5, 1)])
+def test_fp64_adjoint_matches_oracle_adjoint(mods, T, K):
+    torch, ag = mods
+    print_case("adjoint-fp64-vs-oracle", T=T, K=K)
+    shape = (6, 5, 4)
+    u0, F, ut = problem(shape)
+    L, gu, gF, _ = gpu_grad(torch, ag, shape, 0.6, T, K, "f64", u0, F, ut)
+    Lr, gur, gFr = oracle.inverse_grad(u0, F, ut, 0.6, T)
+    assert L == pytest.approx(Lr, rel=1e-12)
+    assert field_scaled_err(gu, gur) < 1e-10
+    assert field_scaled_err(gF, gFr) < 1e-10
+
+
+def test_fp32_adjoint_matches_oracle_adjoint(mods):
+    torch, ag = mods
+    print_case("adjoint-fp32-vs-oracle")
+    shape = (10, 8, 12)
+    u0, F, ut = problem(shape, seed=7)
+    L, gu, gF, _ = gpu_grad(torch, ag, shape, 0.6, 40, 10, "f32", u0, F, ut)
+    Lr, gur, gFr = oracle.inverse_grad(u0, F, ut, 0.6, 40)
+    assert L == pytest.approx(Lr, rel=1e-5)
+    assert field_scaled_err(gu, gur) < 1e-4
+    assert field_scaled_err(gF, gFr) < 1e-4
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-7), ("f32", 1e-4)])
+def test_adjoint_matches_central_differences(mods, storage, tol):
+    """<grad, d> vs (L(p + eps d) - L(p - eps d)) / 2 eps, eps = 1e-3, FP64 forward."""
+    torch, ag = mods
+    print_case("adjoint-fd", storage=storage)
+    shape = (16, 16, 16)
+    T, K, tau = 60, 10, 0.6
+    u0, F, ut = problem(shape, seed=3, famp=0.0)
+    _, gu, gF, _ = gpu_grad(torch, ag, shape, tau, T, K, storage, u0, F, ut)
+    fwd = ag.DifferentiableLBM(shape, tau, T, checkpoint_every=K, storage="f64")
+    utt = torch.tensor(ut, device="cuda")
+
+    def loss(a, b):
+        u = fwd.forward_raw(torch.tensor(a, device="cuda"), torch.tensor(b, device="cuda"))
+        return float(ag.velocity_matching_loss(u, utt))
+
+    rng = np.random.default_rng(42)
+    eps = 1e-3
+    for k in range(4):
+        du = rng.standard_normal(u0.shape)
+        dF = 1e-3 * rng.standard_normal(F.shape)  # forces live on a 1e-3 smaller scale
+        fd = (loss(u0 + eps * du, F + eps * dF) - loss(u0 - eps * du, F - eps * dF)) / (2 * eps)
+        ad = float(np.sum(gu.astype(np.float64) * du) + np.sum(gF.astype(np.float64) * dF))
+        assert abs(ad - fd) <= tol * abs(fd), (k, ad, fd)
+
+
+def test_checkpoint_interval_autotune_and_memory(mods):
+    torch, ag = mods
+    print_case("adjoint-autotune")
+    s = ag.DifferentiableLBM((8, 8, 8), 0.6, 100, checkpoint_every=0)
+    assert s.checkpoint_every == 10
+    assert s.memory_bytes > 0
+    s.close()
+
+
+def test_adjoint_errors(mods):
+    torch, ag = mods
+    from lattice_boltzmann_d3q19_amd import D3Q19Error
+
+    print_case("adjoint-errors")
+    s = ag.DifferentiableLBM((4, 4, 4), 0.6, 5)
+    with pytest.raises(D3Q19Error, match="ESTATE"):
+        s.backward_raw(torch.zeros((3, 4, 4, 4), device="cuda"))
+    with pytest.raises(D3Q19Error, match="EINVAL"):
+        ag.DifferentiableLBM((4, 4, 4), 0.6, 5, storage="f16")
