@@ -50,6 +50,32 @@ def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup):
     return ms, wall, diag
 
 
+def copy_peak_gbs(amd, torch, nbytes=2 << 30, reps=20):
+    """Achievable HBM bandwidth: the library's plain 16 B/lane copy kernel."""
+    import ctypes
+
+    from lattice_boltzmann_d3q19_amd._lib import call
+
+    a = torch.empty(nbytes // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    stream = torch.cuda.current_stream()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    for _ in range(3):
+        call("d3q19_copy_kernel", ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), nbytes, sp)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(reps):
+        call("d3q19_copy_kernel", ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), nbytes, sp)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     """The FP64 CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
     import numpy as np
@@ -102,6 +128,7 @@ def main() -> int:
     rho0, u0 = taylor_green(shape, args.u0)
     nu = (args.tau - 0.5) / 3.0
 
+    copy_gbs = copy_peak_gbs(amd, torch)
     ms, wall, diag = time_lattice(amd, torch, shape, args.tau, "f32", rho0, u0, args.steps, args.warmup)
     glups = ncell * args.steps / (ms * 1e-3) / 1e9
     kernel_ms = ms / args.steps
@@ -143,6 +170,8 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
         },
+        "copy_peak_GBs": round(copy_gbs, 1),
+        "frac_of_copy_peak": round(achieved / copy_gbs, 4),
         "kernel": "d3q19::k_step<TF32, BC_PERIODIC>",
         "algorithmic_bytes_per_cell": bytes_per_cell("f32"),
         "wall_s": round(wall, 3),

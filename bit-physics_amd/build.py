@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB_DIR = HERE / "lib"
 LIB = LIB_DIR / "libd3q19.so"
-SOURCES = ["d3q19_capi.hip", "d3q19_adjoint.hip"]
+SOURCES = ["d3q19_capi.hip", "d3q19_adjoint.hip", "d3q19_probe.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
@@ -39,6 +39,29 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm required to build libd3q19.so)")
 
 
+NODE_DIR = HERE / "node"
+NODE_ADDON = NODE_DIR / "build" / "d3q19.node"
+
+
+def build_node_addon(verbose: bool = False) -> Path | None:
+    """Node-API addon over the C ABI (g++ against the system node headers)."""
+    inc = next((p for p in (Path("/usr/include/node"), Path("/usr/local/include/node"))
+                if (p / "node_api.h").exists()), None)
+    cxx = shutil.which("g++")
+    if inc is None or cxx is None:
+        if verbose:
+            print("node headers or g++ not found: skipping the Node-API addon", file=sys.stderr)
+        return None
+    NODE_ADDON.parent.mkdir(exist_ok=True)
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-shared", "-fPIC", f"-I{inc}",
+           "-DNODE_GYP_MODULE_NAME=d3q19", str(NODE_DIR / "d3q19_addon.cc"), "-o", str(NODE_ADDON),
+           f"-L{LIB_DIR}", "-ld3q19", "-Wl,-rpath,$ORIGIN/../../lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return NODE_ADDON
+
+
 def build(verbose: bool = False) -> Path:
     LIB_DIR.mkdir(exist_ok=True)
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
@@ -48,6 +71,7 @@ def build(verbose: bool = False) -> Path:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_node_addon(verbose)
     return LIB
 
 

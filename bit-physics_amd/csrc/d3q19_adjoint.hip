@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(BLOCK)
 struct d3q19_adjoint {
     d3q19_config cfg;
     int32_t T = 0, K = 0, nck = 0;
-    hipStream_t stream = nullptr, own_stream = nullptr;
+    hipStream_t stream = nullptr;  // nullptr: the default stream
     uint32_t ncell = 0;
     size_t stride = 0, es = 0, cs = 0;
     std::vector<void*> ckpt;  // H_{kK}
@@ -336,15 +336,7 @@ int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void
     A->stride = ((ncell + 255) / 256) * 256 + 256;
     A->es = cfg->storage == D3Q19_F64 ? 8 : 4;
     A->cs = A->es;
-    if (stream) {
-        A->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking) != hipSuccess) {
-            delete A;
-            return set_error(D3Q19_EHIP, "hipStreamCreate failed");
-        }
-        A->stream = A->own_stream;
-    }
+    A->stream = (hipStream_t)stream;  // NULL: the default stream
     const size_t state = (size_t)Q * A->stride * A->es;
     const size_t lam = (size_t)Q * A->stride * A->cs;
     const size_t vec = 3 * (size_t)ncell * A->cs;
@@ -358,7 +350,6 @@ int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void
     ok = ok && hipMalloc(&A->gF, vec) == hipSuccess && hipMalloc(&A->gu, vec) == hipSuccess;
     if (!ok) {
         free_all(A);
-        if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
         delete A;
         return set_error(D3Q19_ENOMEM, "adjoint checkpoint allocation failed");
     }
@@ -369,21 +360,15 @@ int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void
 
 int d3q19_adjoint_destroy(d3q19_adjoint* A) {
     if (!A) return D3Q19_OK;
-    if (A->stream) (void)hipStreamSynchronize(A->stream);
+    (void)hipStreamSynchronize(A->stream);
     free_all(A);
-    if (A->own_stream) (void)hipStreamDestroy(A->own_stream);
     delete A;
     return D3Q19_OK;
 }
 
 int d3q19_adjoint_set_stream(d3q19_adjoint* A, void* stream) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
-    A->stream = stream ? (hipStream_t)stream : A->own_stream;
-    if (!A->stream) {
-        if (hipStreamCreateWithFlags(&A->own_stream, hipStreamNonBlocking) != hipSuccess)
-            return set_error(D3Q19_EHIP, "hipStreamCreate failed");
-        A->stream = A->own_stream;
-    }
+    A->stream = (hipStream_t)stream;
     return D3Q19_OK;
 }
 

@@ -58,8 +58,7 @@ struct DevBuf {
 
 struct d3q19_lattice {
     d3q19_config cfg;
-    hipStream_t stream = nullptr;
-    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;  // nullptr: the default stream
     uint32_t ncell = 0;
     size_t stride = 0;  // elements between direction planes
     size_t esize = 0;
@@ -72,6 +71,7 @@ struct d3q19_lattice {
     double* dout = nullptr;  // device results of reductions
     float* uscratch = nullptr;
     bool has_state = false;
+    bool local = false;  // buffer cur holds pre-collision f^n in place
     int64_t steps = 0;
     size_t bytes = 0;
 };
@@ -99,6 +99,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.force = (const C*)L->force;
     p.brink = (const C*)L->brink;
     p.solid = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->solid : nullptr;
+    p.local = L->local ? 1 : 0;
     return p;
 }
 
@@ -122,23 +123,29 @@ CPlanes<Tag> cplanes(const d3q19_lattice* L, int which) {
 template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
-    const Params<C> p = make_params<C>(L);
     const uint32_t g = nblocks(L->ncell);
     for (int32_t s = 0; s < n; ++s) {
+        const Params<C> p = make_params<C>(L);
         CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
         Planes<Tag> out = planes<Tag>(L, L->cur ^ 1);
         float* r = (s == n - 1) ? rho : nullptr;
         float* v = (s == n - 1) ? u : nullptr;
-        switch (L->cfg.boundary) {
-            case D3Q19_BC_PERIODIC:
-                k_step<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                break;
-            case D3Q19_BC_YWALLS:
-                k_step<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                break;
-            default:
-                k_step<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                break;
+        if (L->local) {
+            // first step after a load: collide f^n in place, no streaming
+            k_collide_local<Tag><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+            L->local = false;
+        } else {
+            switch (L->cfg.boundary) {
+                case D3Q19_BC_PERIODIC:
+                    k_step<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                    break;
+                case D3Q19_BC_YWALLS:
+                    k_step<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                    break;
+                default:
+                    k_step<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+                    break;
+            }
         }
         HIPCHK(hipGetLastError());
         L->cur ^= 1;
@@ -151,9 +158,10 @@ template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
-    k_init<Tag, In><<<nblocks(L->ncell), BLOCK, 0, L->stream>>>(f_dev, planes<Tag>(L, L->cur), p);
+    k_load<Tag, In><<<nblocks(L->ncell), BLOCK, 0, L->stream>>>(f_dev, planes<Tag>(L, L->cur), p);
     HIPCHK(hipGetLastError());
     L->has_state = true;
+    L->local = true;
     L->steps = 0;
     return D3Q19_OK;
 }
@@ -348,15 +356,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     // Plane stride: a multiple of 256 elements plus one 256-element pad, so the
     // 19 planes do not start at the same offset modulo large powers of two.
     L->stride = ((ncell + 255) / 256) * 256 + 256;
-    if (stream) {
-        L->stream = (hipStream_t)stream;
-    } else {
-        if (hipStreamCreateWithFlags(&L->own_stream, hipStreamNonBlocking) != hipSuccess) {
-            delete L;
-            return fail(D3Q19_EHIP, "hipStreamCreate failed");
-        }
-        L->stream = L->own_stream;
-    }
+    L->stream = (hipStream_t)stream;  // NULL: the default stream
     const size_t lat_bytes = (size_t)Q * L->stride * es;
     const uint32_t nb = nblocks(ncell);
     if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
@@ -372,23 +372,17 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
 
 int d3q19_destroy(d3q19_lattice* L) {
     if (!L) return D3Q19_OK;
-    if (L->stream) (void)hipStreamSynchronize(L->stream);
+    (void)hipStreamSynchronize(L->stream);
     void* ps[] = {L->buf[0], L->buf[1], L->force, L->brink, L->solid, L->partials, L->dout, L->uscratch};
     for (void* p : ps)
         if (p) (void)hipFree(p);
-    if (L->own_stream) (void)hipStreamDestroy(L->own_stream);
     delete L;
     return D3Q19_OK;
 }
 
 int d3q19_set_stream(d3q19_lattice* L, void* stream) {
     if (int e = check_lat(L)) return e;
-    L->stream = stream ? (hipStream_t)stream : L->own_stream;
-    if (!L->stream) {
-        if (hipStreamCreateWithFlags(&L->own_stream, hipStreamNonBlocking) != hipSuccess)
-            return fail(D3Q19_EHIP, "hipStreamCreate failed");
-        L->stream = L->own_stream;
-    }
+    L->stream = (hipStream_t)stream;
     return D3Q19_OK;
 }
 
@@ -396,8 +390,6 @@ int d3q19_set_populations(d3q19_lattice* L, const void* f, int32_t dt, int32_t o
     if (int e = check_lat(L)) return e;
     if (int e = check_dtype(dt)) return e;
     if (!f) return fail(D3Q19_EINVAL, "null populations");
-    if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
-        return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
     const size_t n = (size_t)Q * L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf tmp;

@@ -109,6 +109,7 @@ template <class C> struct Params {
     const uint8_t* solid;   // (ncell) solid flag or null
     C face_u[6][3];         // half-way wall velocities
     C wtop[3], wbot[3];     // reference y-wall velocities
+    int32_t local;          // state holds pre-collision f^n in place (not yet streamed)
 };
 
 template <class T>
@@ -294,9 +295,10 @@ __device__ __forceinline__ void readback_moments(const C d[Q], const Params<C>& 
     C fx, fy, fz;
     cell_force(p, cell, mx, my, mz, fx, fy, fz);
     const C rs = rho > C(1e-30) ? rho : C(1e-30);
-    ux = (mx + C(0.5) * fx) / rs;
-    uy = (my + C(0.5) * fy) / rs;
-    uz = (mz + C(0.5) * fz) / rs;
+    const C irs = C(1) / rs;  // same operation order as collide()
+    ux = (mx + C(0.5) * fx) * irs;
+    uy = (my + C(0.5) * fy) * irs;
+    uz = (mz + C(0.5) * fz) * irs;
     rho_o = rho;
 }
 

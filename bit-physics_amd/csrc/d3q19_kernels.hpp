@@ -49,27 +49,64 @@ static __global__ void __launch_bounds__(BLOCK)
     }
 }
 
-// Collide a reference-layout population field f^n (no streaming) into the
-// engine's post-collision storage: the state the fused step consumes.
+// Load a reference-layout population field f^n as the lattice state, in
+// place (pre-collision form): stored deviations d_i = f_i - w_i.
 template <class Tag, class In>
 static __global__ void __launch_bounds__(BLOCK)
-    k_init(const In* __restrict__ f, Planes<Tag> out, Params<typename Codec<Tag>::C> p) {
+    k_load(const In* __restrict__ f, Planes<Tag> out, Params<typename Codec<Tag>::C> p) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    if (cell >= p.ncell) return;
+    const bool solid = p.solid && p.solid[cell];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const C d = solid ? C(0) : (C)((double)f[(size_t)i * p.ncell + cell] - wgt<double>(i));
+        out.p[i][cell] = Codec<Tag>::enc(d, i);
+    }
+}
+
+// First step after a load: collide the in-place f^n (no streaming) into the
+// post-collision storage the fused step consumes; optionally write rho/u of f^n.
+template <class Tag>
+static __global__ void __launch_bounds__(BLOCK)
+    k_collide_local(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
+                    float* u_out) {
     typedef typename Codec<Tag>::C C;
     const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
     if (cell >= p.ncell) return;
     C d[Q];
+    C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
     if (p.solid && p.solid[cell]) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) out.p[i][cell] = Codec<Tag>::enc(C(0), i);
-        return;
+        for (int i = 0; i < Q; ++i) d[i] = C(0);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d[i] = Codec<Tag>::dec(in.p[i][cell], i);
+        collide(d, p, cell, rho, ux, uy, uz);
     }
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-        d[i] = (C)((double)f[(size_t)i * p.ncell + cell] - wgt<double>(i));
-    C rho, ux, uy, uz;
-    collide(d, p, cell, rho, ux, uy, uz);
-#pragma unroll
     for (int i = 0; i < Q; ++i) out.p[i][cell] = Codec<Tag>::enc(d[i], i);
+    if (rho_out) rho_out[cell] = (float)rho;
+    if (u_out) {
+        u_out[cell] = (float)ux;
+        u_out[p.ncell + cell] = (float)uy;
+        u_out[2 * p.ncell + cell] = (float)uz;
+    }
+}
+
+// Pre-collision deviations of the current state at a cell: in place after a
+// load, else pull + boundary of the stored post-collision state.
+template <class Tag, int BC>
+__device__ __forceinline__ void state_at(const CPlanes<Tag>& in, const Params<typename Codec<Tag>::C>& p,
+                                         uint32_t cell, typename Codec<Tag>::C d[Q]) {
+    if (p.local) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d[i] = Codec<Tag>::dec(in.p[i][cell], i);
+    } else {
+        uint32_t x, y, z;
+        cell_coords(cell, p.ny, p.nz, x, y, z);
+        pull<Tag, BC>(in, p, x, y, z, cell, d);
+    }
 }
 
 // Read-back: pull + boundary of the stored post-collision state gives the
@@ -81,15 +118,13 @@ static __global__ void __launch_bounds__(BLOCK)
     typedef typename Codec<Tag>::C C;
     const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
     if (cell >= p.ncell) return;
-    uint32_t x, y, z;
-    cell_coords(cell, p.ny, p.nz, x, y, z);
     C d[Q];
     const bool solid = (BC == BC_HALFWAY) && p.solid && p.solid[cell];
     if (solid) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-        pull<Tag, BC>(in, p, x, y, z, cell, d);
+        state_at<Tag, BC>(in, p, cell, d);
     }
     if (f_out) {
 #pragma unroll
@@ -120,10 +155,8 @@ static __global__ void __launch_bounds__(BLOCK)
     const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
     double v[NDIAG] = {0, 0, 0, 0, 0};
     if (cell < p.ncell && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
-        uint32_t x, y, z;
-        cell_coords(cell, p.ny, p.nz, x, y, z);
         C d[Q];
-        pull<Tag, BC>(in, p, x, y, z, cell, d);
+        state_at<Tag, BC>(in, p, cell, d);
         C drho, mx, my, mz, rho, ux, uy, uz;
         moments(d, drho, mx, my, mz);
         readback_moments(d, p, cell, rho, ux, uy, uz);

@@ -33,7 +33,7 @@ EXPORTED = (
     "d3q19_feq_field", "d3q19_density_field", "d3q19_momentum_field",
     "d3q19_macroscopic_velocity", "d3q19_adjoint_create", "d3q19_adjoint_destroy",
     "d3q19_adjoint_set_stream", "d3q19_adjoint_forward", "d3q19_adjoint_backward",
-    "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes",
+    "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
 )
 
 
@@ -116,6 +116,7 @@ def load() -> ctypes.CDLL:
         "d3q19_adjoint_backward": (ctypes.c_int, [P, P, P, P, i32, i32]),
         "d3q19_adjoint_checkpoint_every": (i32, [P]),
         "d3q19_adjoint_memory_bytes": (ctypes.c_size_t, [P]),
+        "d3q19_copy_kernel": (ctypes.c_int, [P, P, ctypes.c_size_t, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

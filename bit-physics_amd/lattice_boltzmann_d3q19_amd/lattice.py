@@ -221,7 +221,8 @@ class Lattice:
         return self
 
     def step_moments(self, n: int, rho_out=None, u_out=None):
-        """Advance n steps; the last pass writes rho/u into FP32 device tensors."""
+        """Advance n steps; the last step's pass also writes rho/u (FP32 device
+        tensors) of the state that step started from."""
         self._sync_stream()
         call("d3q19_step_moments", self._h, int(n), dev.ptr(rho_out), dev.ptr(u_out))
         return self

@@ -1,0 +1,49 @@
+// TypeScript declarations of the D3Q19 Node-API addon (libd3q19 over its C ABI).
+// Populations are (19, Nx, Ny, Nz) z-fastest in the reference's lexicographic
+// velocity order; vector fields are (3, Nx, Ny, Nz).
+
+export type Storage = 'f64' | 'f32' | 'f16' | 'bf16';
+export type Boundary = 'periodic' | 'ywalls' | 'halfway';
+export type Face = '-x' | '+x' | '-y' | '+y' | '-z' | '+z';
+export type Vec3 = [number, number, number];
+
+export interface LatticeConfig {
+  nx: number;
+  ny: number;
+  nz: number;
+  tau: number;
+  storage?: Storage;            // default 'f32'
+  boundary?: Boundary;          // default 'periodic'
+  wallVelocityTop?: Vec3;       // 'ywalls': apply_bounce_back_y_walls(wall_velocity_top)
+  wallVelocityBottom?: Vec3;    // 'ywalls': apply_bounce_back_y_walls(wall_velocity_bottom)
+  wallFaces?: Face[];           // 'halfway': wall faces (others periodic)
+  faceVelocity?: Partial<Record<Face, Vec3>>;
+  force?: Vec3;                 // uniform Guo body force
+  device?: number;
+}
+
+export interface Moments {
+  rho: Float64Array;            // (Nx, Ny, Nz)
+  u: Float64Array;              // (3, Nx, Ny, Nz), Guo-corrected
+}
+
+export interface Diagnostics {
+  mass: number;
+  momentum: Vec3;
+  kineticEnergy: number;
+}
+
+export declare class Lattice {
+  constructor(cfg: LatticeConfig);
+  readonly shape: Vec3;
+  setEquilibrium(rho: Float64Array | Float32Array | null, u: Float64Array | Float32Array): void;
+  setPopulations(f: Float64Array | Float32Array): void;
+  step(n?: number): void;
+  readPopulations(dtype?: 'f64' | 'f32'): Float64Array | Float32Array;
+  readMoments(): Moments;
+  diagnostics(): Diagnostics;
+  stepCount(): number;
+  close(): void;
+}
+
+export declare function version(): string;

@@ -1,0 +1,67 @@
+'use strict';
+// GPU check of the Node-API addon: the reference's published Poiseuille and
+// Couette known answers (docs/sim-specs/lattice/lattice-boltzmann-d3q19/equivalence.md:240-243)
+// and read-back consistency.  Prints one JSON line; exits non-zero on failure.
+const assert = require('assert');
+const { Lattice, version } = require('./index.js');
+
+function maxSpeed(u, n) {
+  let m = 0;
+  for (let c = 0; c < n; ++c) {
+    const s = Math.hypot(u[c], u[n + c], u[2 * n + c]);
+    if (s > m) m = s;
+  }
+  return m;
+}
+
+const out = { version: version() };
+
+{
+  const nx = 64, ny = 32, nz = 3, n = nx * ny * nz;
+  const lat = new Lattice({ nx, ny, nz, tau: 0.7, storage: 'f64', boundary: 'ywalls', force: [1e-5, 0, 0] });
+  lat.setEquilibrium(null, new Float64Array(3 * n));
+  lat.step(1000);
+  const { rho, u } = lat.readMoments();
+  out.poiseuille_umax = maxSpeed(u, n).toExponential(7);
+  out.poiseuille_rhomax = Math.max(...rho).toExponential(7);
+  assert.strictEqual(out.poiseuille_umax, '8.6508320e-3');
+  assert.strictEqual(out.poiseuille_rhomax, '1.0000000e+0');
+  assert.strictEqual(lat.stepCount(), 1000);
+  const f = lat.readPopulations('f64');
+  assert.strictEqual(f.length, 19 * n);
+  let mass = 0;
+  for (let i = 0; i < f.length; ++i) mass += f[i];
+  const d = lat.diagnostics();
+  assert(Math.abs(mass - d.mass) < 1e-9 * n, 'diagnostics mass vs read-back sum');
+  lat.close();
+}
+{
+  const nx = 32, ny = 16, nz = 3, n = nx * ny * nz;
+  const lat = new Lattice({ nx, ny, nz, tau: 0.7, storage: 'f64', boundary: 'ywalls', wallVelocityTop: [0.05, 0, 0] });
+  lat.setEquilibrium(null, new Float64Array(3 * n));
+  lat.step(500);
+  out.couette_umax = maxSpeed(lat.readMoments().u, n).toExponential(7);
+  assert.strictEqual(out.couette_umax, '5.0000000e-2');
+  lat.close();
+}
+{
+  // set/read round trip in f32 storage: a rest state is a fixed point
+  const n = 8 * 8 * 8;
+  const lat = new Lattice({ nx: 8, ny: 8, nz: 8, tau: 0.6, boundary: 'halfway', wallFaces: ['-x', '+x', '-y', '+y', '-z', '+z'] });
+  lat.setEquilibrium(null, new Float32Array(3 * n));
+  lat.step(10);
+  const f = lat.readPopulations('f32');
+  const w = [1 / 3, 1 / 18, 1 / 36];
+  let err = 0;
+  for (let i = 0; i < 19; ++i) {
+    const wi = i === 0 ? w[0] : (i <= 6 ? w[1] : w[2]);
+    for (let c = 0; c < n; ++c) err = Math.max(err, Math.abs(f[i * n + c] - wi));
+  }
+  out.rest_fixed_point_err = err;
+  assert(err < 1e-7);
+  let threw = false;
+  try { lat.step(-1); } catch (e) { threw = /EINVAL|status 1/.test(String(e)); }
+  assert(threw, 'negative step count must throw');
+  lat.close();
+}
+console.log(JSON.stringify(out));

@@ -15,8 +15,8 @@
  *    kept on the GPU as f_i - w_i, collision in FP32).
  *  - on_device != 0: the pointer is device memory on the lattice's device;
  *    otherwise host memory (the library stages it).
- *  - stream: a hipStream_t passed as void*; NULL selects the library's own
- *    stream.  Calls are stream-ordered; read_* and diagnostics synchronise.
+ *  - stream: a hipStream_t passed as void*; NULL is HIP's default stream.
+ *    Calls are stream-ordered; read_* and diagnostics synchronise.
  *  - Every function returns a d3q19_status; d3q19_last_error() gives the
  *    message of the last failure on the calling thread.  No C++ exception
  *    crosses the ABI.  A handle must not be used by two threads at once.
@@ -98,8 +98,9 @@ int d3q19_set_solid(d3q19_lattice* lat, const uint8_t* solid, int32_t on_device)
 int d3q19_set_brinkman(d3q19_lattice* lat, const void* s, int32_t dtype, int32_t on_device);
 /* Advance n steps. */
 int d3q19_step(d3q19_lattice* lat, int32_t n);
-/* Advance n >= 1 steps and write rho (Nx,Ny,Nz) and u (3,Nx,Ny,Nz) of the new
-   state from the last step's own pass (device FP32 pointers, either may be NULL). */
+/* Advance n >= 1 steps; the last step's own pass also writes rho (Nx,Ny,Nz)
+   and u (3,Nx,Ny,Nz) of the state that step started from (f^{N-1} when the
+   lattice ends at f^N): device FP32 pointers, either may be NULL. */
 int d3q19_step_moments(d3q19_lattice* lat, int32_t n, float* rho_dev, float* u_dev);
 /* Pre-collision populations f^n, as the reference holds them after step n. */
 int d3q19_read_populations(d3q19_lattice* lat, void* f, int32_t dtype, int32_t on_device);
@@ -164,6 +165,12 @@ int d3q19_adjoint_backward(d3q19_adjoint* adj, const void* grad_u_final, void* g
                            void* grad_force, int32_t dtype, int32_t on_device);
 int32_t d3q19_adjoint_checkpoint_every(const d3q19_adjoint* adj);
 size_t d3q19_adjoint_memory_bytes(const d3q19_adjoint* adj);
+
+/* ---- calibration (no reference counterpart) ----------------------------
+ * Plain device copy of `bytes` (multiple of 16) with 16-byte nontemporal
+ * loads/stores, stream-ordered: the measured HBM copy peak and the known
+ * byte count that scales rocprof FETCH_SIZE / WRITE_SIZE. */
+int d3q19_copy_kernel(const void* src, void* dst, size_t bytes, void* stream);
 
 #ifdef __cplusplus
 }

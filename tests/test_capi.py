@@ -95,3 +95,16 @@ def test_create_rejects_bad_config_without_gpu():
     cfg.nx = 0
     assert lib.d3q19_create(ctypes.byref(cfg), None, ctypes.byref(h)) == 1
     assert not h.value
+
+
+def test_node_addon_loads_without_gpu():
+    import shutil as _sh
+
+    node = _sh.which("node")
+    addon = ROOT / "bit-physics_amd" / "node" / "build" / "d3q19.node"
+    if node is None or not addon.exists():
+        pytest.skip("node or the addon is not available")
+    r = subprocess.run([node, "-e", "const m=require(process.argv[1]); console.log(typeof m.Lattice, m.version())",
+                        str(ROOT / "bit-physics_amd" / "node" / "index.js")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("function d3q19")

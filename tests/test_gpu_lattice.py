@@ -199,19 +199,33 @@ def test_fp64_diagnostics_match_oracle(amd, storage):
 
 
 def test_step_moments_equal_readback(amd):
+    """The in-pass moments of a step are those of the state the step started from."""
     import torch
 
     print_case("lattice-step-moments")
     shape = (16, 12, 20)
     f0, F = seeded(shape, seed=8)
-    lat = amd.Lattice(shape, 0.6, storage="f32", force=F)
-    lat.set_populations(f0)
-    rho = torch.empty(shape, dtype=torch.float32, device="cuda")
-    u = torch.empty((3,) + shape, dtype=torch.float32, device="cuda")
-    lat.step_moments(5, rho, u)
-    r2, u2 = lat.moments(dtype=np.float32)
-    assert np.array_equal(rho.cpu().numpy(), r2)
-    assert np.allclose(u.cpu().numpy(), u2, rtol=0, atol=1e-9)
+    for first in (1, 4):  # the first step (in-place collide) and a fused step
+        lat = amd.Lattice(shape, 0.6, storage="f32", force=F)
+        lat.set_populations(f0).step(first - 1)
+        r_before, u_before = lat.moments(dtype=np.float32)
+        rho = torch.empty(shape, dtype=torch.float32, device="cuda")
+        u = torch.empty((3,) + shape, dtype=torch.float32, device="cuda")
+        lat.step_moments(1, rho, u)
+        torch.cuda.synchronize()
+        assert np.array_equal(rho.cpu().numpy(), r_before), first
+        assert np.array_equal(u.cpu().numpy(), u_before), first
+
+
+def test_readback_before_any_step_returns_loaded_state(amd):
+    print_case("lattice-readback-step0")
+    shape = (5, 6, 7)
+    f0, _ = seeded(shape, seed=4)
+    for storage, tol in (("f64", 1e-15), ("f32", 1e-8)):
+        lat = amd.Lattice(shape, 0.7, storage=storage, boundary="ywalls")
+        lat.set_populations(f0)
+        assert np.max(np.abs(lat.populations() - f0)) < tol
+        assert lat.step_count == 0
 
 
 def test_deterministic_bitwise(amd):

@@ -1,0 +1,28 @@
+"""The Node-API addon (bit-physics_amd/node) on the GPU: the reference's
+published Poiseuille / Couette maxima through the TypeScript-facing API."""
+
+from __future__ import annotations
+
+import json
+import shutil
+import subprocess
+
+import pytest
+
+from conftest import PKG_DIR, print_case
+
+pytestmark = pytest.mark.gpu
+
+
+def test_node_addon_known_answers():
+    node = shutil.which("node")
+    if node is None:
+        pytest.skip("node is not installed")
+    addon = PKG_DIR / "node" / "build" / "d3q19.node"
+    assert addon.exists(), "Node-API addon not built"
+    print_case("node-addon")
+    r = subprocess.run([node, str(PKG_DIR / "node" / "test.js")], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["poiseuille_umax"] == "8.6508320e-3"
+    assert out["couette_umax"] == "5.0000000e-2"
