@@ -201,8 +201,7 @@ Params<C> adj_params(const d3q19_adjoint* A) {
     p.ny = A->cfg.ny;
     p.nz = A->cfg.nz;
     p.ncell = A->ncell;
-    p.omega = (C)(1.0 / A->cfg.tau);
-    p.pref = (C)(1.0 - 0.5 / A->cfg.tau);
+    set_collision_constants(p, A->cfg.tau);
     p.force_mode = FORCE_FIELD;
     p.force = (const C*)A->force;
     return p;
@@ -228,7 +227,7 @@ template <class Tag>
 int step_into(d3q19_adjoint* A, void* in, void* out) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = adj_params<C>(A);
-    k_step<Tag, BC_PERIODIC><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
+    k_step<Tag, BC_PERIODIC, true><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
         as_cplanes<Tag>(A, in), as_planes<Tag>(A, out), p, nullptr, nullptr);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;

@@ -86,8 +86,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.ny = (uint32_t)L->cfg.ny;
     p.nz = (uint32_t)L->cfg.nz;
     p.ncell = L->ncell;
-    p.omega = (C)(1.0 / L->cfg.tau);
-    p.pref = (C)(1.0 - 0.5 / L->cfg.tau);
+    set_collision_constants(p, L->cfg.tau);
     p.force_mode = L->cfg.force_mode;
     p.wall_mask = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->cfg.wall_faces : 0u;
     for (int a = 0; a < 3; ++a) {
@@ -120,12 +119,55 @@ CPlanes<Tag> cplanes(const d3q19_lattice* L, int which) {
     return pl;
 }
 
+// Launch shape of the fused step: (z, y, x) blocks of 256 threads when rows
+// are at least one wavefront long, else a linear grid over cells.
+struct StepGrid {
+    dim3 grid, block;
+    bool g3;
+};
+
+inline StepGrid step_grid(const d3q19_lattice* L) {
+    StepGrid sg;
+    const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
+    if (nz >= 64 && ny <= 65535 && nx <= 65535) {
+        const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
+        const uint32_t by = BLOCK / bz;
+        sg.block = dim3(bz, by, 1);
+        sg.grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+        sg.g3 = true;
+    } else {
+        sg.block = dim3(BLOCK, 1, 1);
+        sg.grid = dim3(nblocks(L->ncell), 1, 1);
+        sg.g3 = false;
+    }
+    return sg;
+}
+
+template <class Tag, bool FORCED>
+void launch_k_step(d3q19_lattice* L, const StepGrid& sg, CPlanes<Tag> in, Planes<Tag> out,
+                   const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_step<Tag, BC_PERIODIC, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_step<Tag, BC_YWALLS, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            break;
+        default:
+            k_step<Tag, BC_HALFWAY, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            break;
+    }
+}
+
 template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
     const uint32_t g = nblocks(L->ncell);
+    const StepGrid sg = step_grid(L);
     for (int32_t s = 0; s < n; ++s) {
-        const Params<C> p = make_params<C>(L);
+        Params<C> p = make_params<C>(L);
+        p.grid3d = sg.g3 ? 1 : 0;
+        const bool forced = p.force_mode != D3Q19_FORCE_NONE || p.brink != nullptr;
         CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
         Planes<Tag> out = planes<Tag>(L, L->cur ^ 1);
         float* r = (s == n - 1) ? rho : nullptr;
@@ -134,18 +176,10 @@ int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
             // first step after a load: collide f^n in place, no streaming
             k_collide_local<Tag><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
             L->local = false;
+        } else if (forced) {
+            launch_k_step<Tag, true>(L, sg, in, out, p, r, v);
         } else {
-            switch (L->cfg.boundary) {
-                case D3Q19_BC_PERIODIC:
-                    k_step<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                    break;
-                case D3Q19_BC_YWALLS:
-                    k_step<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                    break;
-                default:
-                    k_step<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
-                    break;
-            }
+            launch_k_step<Tag, false>(L, sg, in, out, p, r, v);
         }
         HIPCHK(hipGetLastError());
         L->cur ^= 1;

@@ -110,11 +110,42 @@ template <class C> struct Params {
     C face_u[6][3];         // half-way wall velocities
     C wtop[3], wbot[3];     // reference y-wall velocities
     int32_t local;          // state holds pre-collision f^n in place (not yet streamed)
+    int32_t grid3d;         // k_step launched on a (z, y, x) grid instead of a linear one
+    // collision coefficients per weight class k (rest, face, edge), each the
+    // correctly rounded value of its exact product: om w_k, 3 om w_k,
+    // 4.5 om w_k, 3 pref w_k, 9 pref w_k
+    C k_base[3], k_odd[3], k_quad[3], k_fodd[3], k_fquad[3];
 };
 
-template <class T>
-__device__ __forceinline__ T ldg(const T* p) {
-    return __builtin_nontemporal_load(p);
+// Fill the collision coefficients of p for relaxation time tau (host).
+template <class C>
+inline void set_collision_constants(Params<C>& p, double tau) {
+    const double om = 1.0 / tau, pr = 1.0 - 0.5 / tau;
+    p.omega = (C)om;
+    p.pref = (C)pr;
+    const double w[3] = {1.0 / 3.0, 1.0 / 18.0, 1.0 / 36.0};
+    for (int k = 0; k < 3; ++k) {
+        p.k_base[k] = (C)(om * w[k]);
+        p.k_odd[k] = (C)(3.0 * om * w[k]);
+        p.k_quad[k] = (C)(4.5 * om * w[k]);
+        p.k_fodd[k] = (C)(3.0 * pr * w[k]);
+        p.k_fquad[k] = (C)(9.0 * pr * w[k]);
+    }
+}
+
+// Population i at byte offset `off` of its plane.  Offsets are 32-bit (the
+// lattice keeps ncell * 8 < 2^32), so loads address as SGPR plane base +
+// 32-bit VGPR offset, with no 64-bit arithmetic per access.
+template <class Tag>
+__device__ __forceinline__ typename Codec<Tag>::C ld(const CPlanes<Tag>& in, int i, uint32_t off) {
+    typedef typename Codec<Tag>::raw R;
+    return Codec<Tag>::dec(*(const R*)((const char*)in.p[i] + off), i);
+}
+
+template <class Tag>
+__device__ __forceinline__ void st(const Planes<Tag>& out, int i, uint32_t off, typename Codec<Tag>::C v) {
+    typedef typename Codec<Tag>::raw R;
+    __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)((char*)out.p[i] + off));
 }
 
 // ---------------------------------------------------------------------------
@@ -127,19 +158,22 @@ __device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typena
                                      uint32_t x, uint32_t y, uint32_t z, uint32_t cell,
                                      typename Codec<Tag>::C d[Q]) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t nyz = p.ny * p.nz;
-    const uint32_t xm = (x == 0 ? p.nx - 1 : x - 1) * nyz, x0 = x * nyz,
-                   xp = (x + 1 == p.nx ? 0 : x + 1) * nyz;
-    const uint32_t ym = (y == 0 ? p.ny - 1 : y - 1) * p.nz, y0 = y * p.nz,
-                   yp = (y + 1 == p.ny ? 0 : y + 1) * p.nz;
-    const uint32_t zm = (z == 0 ? p.nz - 1 : z - 1), z0 = z, zp = (z + 1 == p.nz ? 0 : z + 1);
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
+    const uint32_t nyzb = p.ny * p.nz * ES, nzb = p.nz * ES;
+    const uint32_t xm = (x == 0 ? p.nx - 1 : x - 1) * nyzb, x0 = x * nyzb,
+                   xp = (x + 1 == p.nx ? 0 : x + 1) * nyzb;
+    const uint32_t ym = (y == 0 ? p.ny - 1 : y - 1) * nzb, y0 = y * nzb,
+                   yp = (y + 1 == p.ny ? 0 : y + 1) * nzb;
+    const uint32_t zm = (z == 0 ? p.nz - 1 : z - 1) * ES, z0 = z * ES,
+                   zp = (z + 1 == p.nz ? 0 : z + 1) * ES;
+    const uint32_t self = cell * ES;
     if constexpr (BC != BC_HALFWAY) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
             const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
             const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
-            d[i] = Codec<Tag>::dec(in.p[i][sx + sy + sz], i);
+            d[i] = ld(in, i, sx + sy + sz);
         }
     }
     if constexpr (BC == BC_YWALLS) {
@@ -187,16 +221,16 @@ __device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typena
             if (face >= 0) {
                 C cu = C(CX[i]) * p.face_u[face][0] + C(CY[i]) * p.face_u[face][1] +
                        C(CZ[i]) * p.face_u[face][2];
-                d[i] = Codec<Tag>::dec(in.p[OPP[i]][cell], OPP[i]) + C(6) * wgt<C>(i) * cu;
+                d[i] = ld(in, OPP[i], self) + C(6) * wgt<C>(i) * cu;
             } else {
                 const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
                 const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
                 const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
                 const uint32_t src = sx + sy + sz;
-                if (p.solid && p.solid[src])
-                    d[i] = Codec<Tag>::dec(in.p[OPP[i]][cell], OPP[i]);
+                if (p.solid && p.solid[src / ES])
+                    d[i] = ld(in, OPP[i], self);
                 else
-                    d[i] = Codec<Tag>::dec(in.p[i][src], i);
+                    d[i] = ld(in, i, src);
             }
         }
     }
@@ -250,34 +284,70 @@ __device__ __forceinline__ void cell_force(const Params<C>& p, uint32_t cell, C 
 //   u = (m + F/2)/rho,  f*_i = f_i - (f_i - feq_i(rho,u))/tau + G_i
 //   feq_i - w_i = w_i (drho + rho (3 cu + 4.5 cu^2 - 1.5 u^2))
 //   G_i = (1 - 1/(2 tau)) w_i [3 (c_i - u).F + 9 (c_i.u)(c_i.F)]
-// Returns rho and the Guo-corrected velocity (macroscopic_velocity, bgk.py:129-142).
-template <class C>
+// evaluated per opposite pair (i, opp(i)): c_opp = -c_i flips the sign of
+// the odd terms (3 cu, 3 c.F) and keeps the even ones.  The collision must
+// leave the mass unchanged and add exactly F to the momentum; in float the
+// rounded coefficients would miss both by a fixed relative amount every step
+// (a systematic drift), so the rest population takes the mass residual and
+// the three axis pairs take the momentum residual.  FORCED = false compiles
+// the force terms out.  Returns rho and the Guo-corrected velocity
+// (macroscopic_velocity, bgk.py:129-142).
+template <class C, bool FORCED = true>
 __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cell, C& rho_o,
                                         C& ux_o, C& uy_o, C& uz_o) {
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
-    C fx, fy, fz;
-    cell_force(p, cell, mx, my, mz, fx, fy, fz);
-    const bool forced = (p.force_mode != FORCE_NONE) || (p.brink != nullptr);
+    C fx = C(0), fy = C(0), fz = C(0);
+    if constexpr (FORCED) cell_force(p, cell, mx, my, mz, fx, fy, fz);
     const C rs = rho > C(1e-30) ? rho : C(1e-30);
     const C irs = C(1) / rs;
     const C ux = (mx + C(0.5) * fx) * irs;
     const C uy = (my + C(0.5) * fy) * irs;
     const C uz = (mz + C(0.5) * fz) * irs;
-    const C usq15 = C(1.5) * (ux * ux + uy * uy + uz * uz);
+    const C usq = ux * ux + uy * uy + uz * uz;
     const C om = p.omega;
+    // even part without the cu^2 term: om w (drho - 1.5 rho u^2) [- 3 pref w u.F]
+    const C t = drho - C(1.5) * rho * usq;
+    C uf = C(0);
+    if constexpr (FORCED) uf = ux * fx + uy * fy + uz * fz;
+    C base[3], odd[3], quad[3];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
+    for (int k = 1; k < 3; ++k) {
+        base[k] = p.k_base[k] * t;
+        if constexpr (FORCED) base[k] -= p.k_fodd[k] * uf;
+        odd[k] = p.k_odd[k] * rho;
+        quad[k] = p.k_quad[k] * rho;
+    }
+    C even[Q], oddv[Q];
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+        const int k = i <= 6 ? 1 : 2;
         const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
-        const C deq = wgt<C>(i) * (drho + rho * (C(3) * cu + C(4.5) * cu * cu - usq15));
-        C v = d[i] + om * (deq - d[i]);
-        if (forced) {
+        even[i] = base[k] + quad[k] * cu * cu;
+        oddv[i] = odd[k] * cu;
+        if constexpr (FORCED) {
             const C cf = C(CX[i]) * fx + C(CY[i]) * fy + C(CZ[i]) * fz;
-            const C uf = ux * fx + uy * fy + uz * fz;
-            v += p.pref * wgt<C>(i) * (C(3) * (cf - uf) + C(9) * cu * cf);
+            even[i] += p.k_fquad[k] * cu * cf;
+            oddv[i] += p.k_fodd[k] * cf;
         }
-        d[i] = v;
+    }
+    // momentum residual: sum_i c_i (om feq_i + G_i) = om m + F
+    const C hx = C(0.5) * (om * mx + fx), hy = C(0.5) * (om * my + fy), hz = C(0.5) * (om * mz + fz);
+    oddv[1] = hx - (oddv[7] + oddv[9] + oddv[11] + oddv[13]);
+    oddv[3] = hy - (oddv[7] - oddv[9] + oddv[15] + oddv[17]);
+    oddv[5] = hz - (oddv[11] - oddv[13] + oddv[15] - oddv[17]);
+    // mass residual: sum_i (om feq_i + G_i) = om drho
+    C sum_even = C(0);
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) sum_even += even[i];
+    const C rest = om * drho - C(2) * sum_even;
+    // relaxation as d + (om feq + G) - om d, one fma per population
+    d[0] = fma(-om, d[0], d[0] + rest);
+#pragma unroll
+    for (int i = 1; i < Q; i += 2) {
+        d[i] = fma(-om, d[i], d[i] + (even[i] + oddv[i]));
+        d[i + 1] = fma(-om, d[i + 1], d[i + 1] + (even[i] - oddv[i]));
     }
     rho_o = rho;
     ux_o = ux;

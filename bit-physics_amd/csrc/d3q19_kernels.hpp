@@ -12,20 +12,30 @@ constexpr int BLOCK = 256;
 // Reads the post-collision state of step n, writes that of step n+1, and
 // (optionally) rho and u of the reference state f^{n+1} from the same pass.
 // ---------------------------------------------------------------------------
-template <class Tag, int BC>
+template <class Tag, int BC, bool FORCED>
 static __global__ void __launch_bounds__(BLOCK)
     k_step(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
            float* u_out) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
-    uint32_t x, y, z;
-    cell_coords(cell, p.ny, p.nz, x, y, z);
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
+    uint32_t x, y, z, cell;
+    if (p.grid3d) {
+        // blocks tile (z, y) rows: z fastest, no integer division
+        z = blockIdx.x * blockDim.x + threadIdx.x;
+        y = blockIdx.y * blockDim.y + threadIdx.y;
+        x = blockIdx.z;
+        if (z >= p.nz || y >= p.ny) return;
+        cell = (x * p.ny + y) * p.nz + z;
+    } else {
+        cell = blockIdx.x * BLOCK + threadIdx.x;
+        if (cell >= p.ncell) return;
+        cell_coords(cell, p.ny, p.nz, x, y, z);
+    }
+    const uint32_t self = cell * ES;
     if constexpr (BC == BC_HALFWAY) {
         if (p.solid && p.solid[cell]) {
 #pragma unroll
-            for (int i = 0; i < Q; ++i)
-                __builtin_nontemporal_store(Codec<Tag>::enc(C(0), i), &out.p[i][cell]);
+            for (int i = 0; i < Q; ++i) st(out, i, self, C(0));
             if (rho_out) rho_out[cell] = 1.0f;
             if (u_out) {
                 u_out[cell] = 0.0f;
@@ -38,9 +48,9 @@ static __global__ void __launch_bounds__(BLOCK)
     C d[Q];
     pull<Tag, BC>(in, p, x, y, z, cell, d);
     C rho, ux, uy, uz;
-    collide(d, p, cell, rho, ux, uy, uz);
+    collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) __builtin_nontemporal_store(Codec<Tag>::enc(d[i], i), &out.p[i][cell]);
+    for (int i = 0; i < Q; ++i) st(out, i, self, d[i]);
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;
