@@ -227,7 +227,7 @@ template <class Tag>
 int step_into(d3q19_adjoint* A, void* in, void* out) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = adj_params<C>(A);
-    k_step<Tag, BC_PERIODIC, true><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
+    k_step<Tag, BC_PERIODIC, true><<<grid_for(A->ncell, step_block<Tag>()), step_block<Tag>(), 0, A->stream>>>(
         as_cplanes<Tag>(A, in), as_planes<Tag>(A, out), p, nullptr, nullptr);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;

@@ -126,18 +126,20 @@ struct StepGrid {
     bool g3;
 };
 
-inline StepGrid step_grid(const d3q19_lattice* L) {
+template <class Tag>
+StepGrid step_grid(const d3q19_lattice* L) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
+    const uint32_t block = step_block<Tag>();
     if (nz >= 64 && ny <= 65535 && nx <= 65535) {
         const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
-        const uint32_t by = BLOCK / bz;
+        const uint32_t by = block / bz;
         sg.block = dim3(bz, by, 1);
         sg.grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
         sg.g3 = true;
     } else {
-        sg.block = dim3(BLOCK, 1, 1);
-        sg.grid = dim3(nblocks(L->ncell), 1, 1);
+        sg.block = dim3(block, 1, 1);
+        sg.grid = dim3((L->ncell + block - 1) / block, 1, 1);
         sg.g3 = false;
     }
     return sg;
@@ -163,7 +165,7 @@ template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
     const uint32_t g = nblocks(L->ncell);
-    const StepGrid sg = step_grid(L);
+    const StepGrid sg = step_grid<Tag>(L);
     for (int32_t s = 0; s < n; ++s) {
         Params<C> p = make_params<C>(L);
         p.grid3d = sg.g3 ? 1 : 0;

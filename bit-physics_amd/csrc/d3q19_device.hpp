@@ -87,6 +87,13 @@ template <> struct Codec<TBF16> {
     }
 };
 
+// Threads per block of the fused step: one 1 KiB row segment per direction
+// and block (256 cells of 4 or 8 bytes, 512 cells of 2 bytes).
+template <class Tag>
+constexpr int step_block() {
+    return sizeof(typename Codec<Tag>::raw) == 2 ? 512 : 256;
+}
+
 // Per-direction base pointers: population i of cell c is base[i][c].  Each
 // base is its own plane, so offsets stay 32-bit even when the lattice is
 // larger than 4 GiB.
@@ -136,10 +143,17 @@ inline void set_collision_constants(Params<C>& p, double tau) {
 // Population i at byte offset `off` of its plane.  Offsets are 32-bit (the
 // lattice keeps ncell * 8 < 2^32), so loads address as SGPR plane base +
 // 32-bit VGPR offset, with no 64-bit arithmetic per access.
+// Pulled values are read exactly once per step: 4-byte populations load
+// with the nontemporal hint (measured faster at 256^3); 2- and 8-byte ones
+// load normally (the hint measured slower for 16-bit storage).
 template <class Tag>
 __device__ __forceinline__ typename Codec<Tag>::C ld(const CPlanes<Tag>& in, int i, uint32_t off) {
     typedef typename Codec<Tag>::raw R;
-    return Codec<Tag>::dec(*(const R*)((const char*)in.p[i] + off), i);
+    const R* a = (const R*)((const char*)in.p[i] + off);
+    if constexpr (sizeof(R) == 4)
+        return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
+    else
+        return Codec<Tag>::dec(*a, i);
 }
 
 template <class Tag>

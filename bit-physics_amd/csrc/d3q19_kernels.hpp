@@ -13,7 +13,7 @@ constexpr int BLOCK = 256;
 // (optionally) rho and u of the reference state f^{n+1} from the same pass.
 // ---------------------------------------------------------------------------
 template <class Tag, int BC, bool FORCED>
-static __global__ void __launch_bounds__(BLOCK)
+static __global__ void __launch_bounds__(step_block<Tag>())
     k_step(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
            float* u_out) {
     typedef typename Codec<Tag>::C C;
@@ -27,7 +27,7 @@ static __global__ void __launch_bounds__(BLOCK)
         if (z >= p.nz || y >= p.ny) return;
         cell = (x * p.ny + y) * p.nz + z;
     } else {
-        cell = blockIdx.x * BLOCK + threadIdx.x;
+        cell = blockIdx.x * blockDim.x + threadIdx.x;
         if (cell >= p.ncell) return;
         cell_coords(cell, p.ny, p.nz, x, y, z);
     }
