@@ -31,38 +31,37 @@ namespace {
 // Initial post-collision "pre-image": H_0 = S^-1 feq(1, u0), i.e.
 // H_0,i(y) = feq_i(1, u0(y + c_i)) - w_i.
 template <class Tag>
-__global__ void __launch_bounds__(BLOCK)
-    k_adj_unstream(const typename Codec<Tag>::C* __restrict__ u0, Planes<Tag> out, uint32_t nx,
-                   uint32_t ny, uint32_t nz) {
+__global__ void __launch_bounds__(step_block<Tag>())
+    k_adj_unstream(const typename Codec<Tag>::C* __restrict__ u0, typename Codec<Tag>::raw* __restrict__ out,
+                   Params<typename Codec<Tag>::C> p) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t ncell = nx * ny * nz;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= ncell) return;
     uint32_t x, y, z;
-    cell_coords(cell, ny, nz, x, y, z);
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t N = p.ncell;
+    C d[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const uint32_t sx = (x + nx + CX[i]) % nx, sy = (y + ny + CY[i]) % ny, sz = (z + nz + CZ[i]) % nz;
-        const uint32_t src = (sx * ny + sy) * nz + sz;
-        const C ux = u0[src], uy = u0[ncell + src], uz = u0[2 * ncell + src];
+        const uint32_t sx = (x + p.nx + CX[i]) % p.nx, sy = (y + p.ny + CY[i]) % p.ny, sz = (z + p.nz + CZ[i]) % p.nz;
+        const uint32_t src = (sx * p.ny + sy) * p.nz + sz;
+        const C ux = u0[src], uy = u0[N + src], uz = u0[2 * N + src];
         const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
-        const C d = wgt<C>(i) * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * (ux * ux + uy * uy + uz * uz));
-        out.p[i][cell] = Codec<Tag>::enc(d, i);
+        d[i] = wgt<C>(i) * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * (ux * ux + uy * uy + uz * uz));
     }
+    store_local<Tag>(out, p, x, y, z, d);
 }
 
 // Seed: lambda^T_j = sum_a g_a (c_ja - u_a) / rho ;  gF_a = g_a / (2 rho).
 template <class Tag>
-__global__ void __launch_bounds__(BLOCK)
-    k_adj_seed(CPlanes<Tag> H, Params<typename Codec<Tag>::C> p, const typename Codec<Tag>::C* __restrict__ g,
-               typename Codec<Tag>::C* __restrict__ lam, size_t lstride, typename Codec<Tag>::C* __restrict__ gF) {
+__global__ void __launch_bounds__(step_block<Tag>())
+    k_adj_seed(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
+               const typename Codec<Tag>::C* __restrict__ g, typename Codec<Tag>::raw* __restrict__ lam,
+               typename Codec<Tag>::C* __restrict__ gF) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
     uint32_t x, y, z;
-    cell_coords(cell, p.ny, p.nz, x, y, z);
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
-    pull<Tag, BC_PERIODIC>(H, p, x, y, z, cell, d);
+    pull<Tag, BC_PERIODIC>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
@@ -72,9 +71,10 @@ __global__ void __launch_bounds__(BLOCK)
     const C ux = (mx + C(0.5) * fx) * ir, uy = (my + C(0.5) * fy) * ir, uz = (mz + C(0.5) * fz) * ir;
     const C gx = g[cell], gy = g[N + cell], gz = g[2 * N + cell];
     const C gu = gx * ux + gy * uy + gz * uz;
+    C l[Q];
 #pragma unroll
-    for (int j = 0; j < Q; ++j)
-        lam[(size_t)j * lstride + cell] = (C(CX[j]) * gx + C(CY[j]) * gy + C(CZ[j]) * gz - gu) * ir;
+    for (int j = 0; j < Q; ++j) l[j] = (C(CX[j]) * gx + C(CY[j]) * gy + C(CZ[j]) * gz - gu) * ir;
+    store_local<Tag>(lam, p, x, y, z, l);
     gF[cell] = C(0.5) * gx * ir;
     gF[N + cell] = C(0.5) * gy * ir;
     gF[2 * N + cell] = C(0.5) * gz * ir;
@@ -82,32 +82,26 @@ __global__ void __launch_bounds__(BLOCK)
 
 // One reverse step: lambda_out = J_C(f^n)^T S^T lambda_in, gF += dC/dF^T S^T lambda_in.
 template <class Tag>
-__global__ void __launch_bounds__(BLOCK)
-    k_adj_step(CPlanes<Tag> H, Params<typename Codec<Tag>::C> p,
-               const typename Codec<Tag>::C* __restrict__ lam_in,
-               typename Codec<Tag>::C* __restrict__ lam_out, size_t lstride,
+__global__ void __launch_bounds__(step_block<Tag>())
+    k_adj_step(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
+               const typename Codec<Tag>::raw* __restrict__ lam_in, typename Codec<Tag>::raw* __restrict__ lam_out,
                typename Codec<Tag>::C* __restrict__ gF) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     uint32_t x, y, z;
-    cell_coords(cell, p.ny, p.nz, x, y, z);
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     const uint32_t N = p.ncell;
-    // S^T: mu_i(x) = lambda_i(x + c_i)
-    const uint32_t nyz = p.ny * p.nz;
-    const uint32_t xm = (x == 0 ? p.nx - 1 : x - 1) * nyz, x0 = x * nyz, xp = (x + 1 == p.nx ? 0 : x + 1) * nyz;
-    const uint32_t ym = (y == 0 ? p.ny - 1 : y - 1) * p.nz, y0 = y * p.nz, yp = (y + 1 == p.ny ? 0 : y + 1) * p.nz;
-    const uint32_t zm = (z == 0 ? p.nz - 1 : z - 1), z0 = z, zp = (z + 1 == p.nz ? 0 : z + 1);
+    // S^T: mu_i(x) = lambda_i(x + c_i)  (the pull run backwards)
+    const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};
+    const uint32_t ys[3] = {y == 0 ? p.ny - 1 : y - 1, y, y + 1 == p.ny ? 0 : y + 1};
+    const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1};
     C mu[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const uint32_t sx = CX[i] > 0 ? xp : (CX[i] < 0 ? xm : x0);
-        const uint32_t sy = CY[i] > 0 ? yp : (CY[i] < 0 ? ym : y0);
-        const uint32_t sz = CZ[i] > 0 ? zp : (CZ[i] < 0 ? zm : z0);
-        mu[i] = lam_in[(size_t)i * lstride + sx + sy + sz];
-    }
+    for (int i = 0; i < Q; ++i)
+        mu[i] = ld<Tag>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
     C d[Q];
-    pull<Tag, BC_PERIODIC>(H, p, x, y, z, cell, d);
+    pull<Tag, BC_PERIODIC>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
@@ -137,29 +131,34 @@ __global__ void __launch_bounds__(BLOCK)
     }
     a_rho *= om;
     const C keep = C(1) - om;
+    C l[Q];
 #pragma unroll
-    for (int j = 0; j < Q; ++j) {
-        const C v = keep * mu[j] + a_rho +
-                    (b[0] * (C(CX[j]) - u[0]) + b[1] * (C(CY[j]) - u[1]) + b[2] * (C(CZ[j]) - u[2])) * ir;
-        lam_out[(size_t)j * lstride + cell] = v;
-    }
+    for (int j = 0; j < Q; ++j)
+        l[j] = keep * mu[j] + a_rho +
+               (b[0] * (C(CX[j]) - u[0]) + b[1] * (C(CY[j]) - u[1]) + b[2] * (C(CZ[j]) - u[2])) * ir;
+    store_local<Tag>(lam_out, p, x, y, z, l);
 #pragma unroll
     for (int a = 0; a < 3; ++a) gF[(size_t)a * N + cell] += ge[a] + C(0.5) * b[a] * ir;
 }
 
 // dL/du0_a = sum_i lambda^0_i w_i (3 c_a + 9 cu c_a - 3 u_a)   (rho = 1)
-template <class C>
-__global__ void __launch_bounds__(BLOCK)
-    k_adj_u0(const C* __restrict__ lam, size_t lstride, const C* __restrict__ u0, C* __restrict__ gu0,
-             uint32_t N) {
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= N) return;
+template <class Tag>
+__global__ void __launch_bounds__(step_block<Tag>())
+    k_adj_u0(const typename Codec<Tag>::raw* __restrict__ lam, Params<typename Codec<Tag>::C> p,
+             const typename Codec<Tag>::C* __restrict__ u0, typename Codec<Tag>::C* __restrict__ gu0) {
+    typedef typename Codec<Tag>::C C;
+    uint32_t x, y, z;
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
+    const uint32_t N = p.ncell;
+    C l[Q];
+    load_local<Tag>(lam, p, x, y, z, l);
     const C u[3] = {u0[cell], u0[N + cell], u0[2 * N + cell]};
     C g[3] = {C(0), C(0), C(0)};
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const C c[3] = {C(CX[i]), C(CY[i]), C(CZ[i])};
-        const C s = lam[(size_t)i * lstride + cell] * wgt<C>(i);
+        const C s = l[i] * wgt<C>(i);
         const C cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
 #pragma unroll
         for (int a = 0; a < 3; ++a) g[a] += s * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]);
@@ -176,7 +175,8 @@ struct d3q19_adjoint {
     int32_t T = 0, K = 0, nck = 0;
     hipStream_t stream = nullptr;  // nullptr: the default stream
     uint32_t ncell = 0;
-    size_t stride = 0, es = 0, cs = 0;
+    size_t es = 0, state_bytes = 0, row_bytes = 0;
+    uint32_t nzp = 0;
     std::vector<void*> ckpt;  // H_{kK}
     std::vector<void*> seg;   // recompute slots (>= 2, also the forward ping-pong)
     void* lam[2] = {nullptr, nullptr};
@@ -201,34 +201,33 @@ Params<C> adj_params(const d3q19_adjoint* A) {
     p.ny = A->cfg.ny;
     p.nz = A->cfg.nz;
     p.ncell = A->ncell;
+    p.nzp = A->nzp;
+    p.dir_bytes = (uint32_t)(A->nzp * A->es);
+    p.row_bytes = A->row_bytes;
     set_collision_constants(p, A->cfg.tau);
     p.force_mode = FORCE_FIELD;
     p.force = (const C*)A->force;
     return p;
 }
 
+// (z, y, x) launch shape shared by every kernel here.
 template <class Tag>
-Planes<Tag> as_planes(const d3q19_adjoint* A, void* base) {
-    Planes<Tag> pl;
-    typedef typename Codec<Tag>::raw R;
-    for (int i = 0; i < Q; ++i) pl.p[i] = (R*)base + (size_t)i * A->stride;
-    return pl;
-}
-
-template <class Tag>
-CPlanes<Tag> as_cplanes(const d3q19_adjoint* A, void* base) {
-    CPlanes<Tag> pl;
-    Planes<Tag> q = as_planes<Tag>(A, base);
-    for (int i = 0; i < Q; ++i) pl.p[i] = q.p[i];
-    return pl;
+void adj_grid(const d3q19_adjoint* A, dim3& grid, dim3& block) {
+    const uint32_t nz = A->cfg.nz;
+    const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
+    const uint32_t by = step_block<Tag>() / bz;
+    block = dim3(bz, by, 1);
+    grid = dim3((nz + bz - 1) / bz, (A->cfg.ny + by - 1) / by, A->cfg.nx);
 }
 
 template <class Tag>
 int step_into(d3q19_adjoint* A, void* in, void* out) {
     typedef typename Codec<Tag>::C C;
+    typedef typename Codec<Tag>::raw R;
     const Params<C> p = adj_params<C>(A);
-    k_step<Tag, BC_PERIODIC, true><<<grid_for(A->ncell, step_block<Tag>()), step_block<Tag>(), 0, A->stream>>>(
-        as_cplanes<Tag>(A, in), as_planes<Tag>(A, out), p, nullptr, nullptr);
+    dim3 g, b;
+    adj_grid<Tag>(A, g, b);
+    k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -236,8 +235,11 @@ int step_into(d3q19_adjoint* A, void* in, void* out) {
 template <class Tag>
 int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
     typedef typename Codec<Tag>::C C;
-    k_adj_unstream<Tag><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
-        (const C*)A->u0, as_planes<Tag>(A, A->ckpt[0]), A->cfg.nx, A->cfg.ny, A->cfg.nz);
+    typedef typename Codec<Tag>::raw R;
+    const Params<C> p = adj_params<C>(A);
+    dim3 g, b;
+    adj_grid<Tag>(A, g, b);
+    k_adj_unstream<Tag><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)A->ckpt[0], p);
     D3Q19_HIPCHK(hipGetLastError());
     void* cur = A->ckpt[0];
     int pp = 0;  // next ping-pong slot
@@ -255,13 +257,12 @@ int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
         cur = out;
     }
     A->Hfinal = cur;
-    const Params<C> p = adj_params<C>(A);
     if (out_dt == D3Q19_F64)
-        k_readback<Tag, BC_PERIODIC, double><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
-            as_cplanes<Tag>(A, cur), p, nullptr, nullptr, (double*)u_out_dev);
+        k_readback<Tag, BC_PERIODIC, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                     (double*)u_out_dev);
     else
-        k_readback<Tag, BC_PERIODIC, float><<<grid_for(A->ncell, BLOCK), BLOCK, 0, A->stream>>>(
-            as_cplanes<Tag>(A, cur), p, nullptr, nullptr, (float*)u_out_dev);
+        k_readback<Tag, BC_PERIODIC, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                    (float*)u_out_dev);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -269,10 +270,11 @@ int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
 template <class Tag>
 int run_backward(d3q19_adjoint* A) {
     typedef typename Codec<Tag>::C C;
+    typedef typename Codec<Tag>::raw R;
     const Params<C> p = adj_params<C>(A);
-    const uint32_t g = grid_for(A->ncell, BLOCK);
-    k_adj_seed<Tag><<<g, BLOCK, 0, A->stream>>>(as_cplanes<Tag>(A, A->Hfinal), p, (const C*)A->gu,
-                                                 (C*)A->lam[0], A->stride, (C*)A->gF);
+    dim3 g, b;
+    adj_grid<Tag>(A, g, b);
+    k_adj_seed<Tag><<<g, b, 0, A->stream>>>((const R*)A->Hfinal, p, (const C*)A->gu, (R*)A->lam[0], (C*)A->gF);
     D3Q19_HIPCHK(hipGetLastError());
     int cur = 0;
     for (int32_t k = A->nck - 1; k >= 0; --k) {
@@ -283,14 +285,13 @@ int run_backward(d3q19_adjoint* A) {
         for (int32_t j = 1; j < len; ++j)
             if (int e = step_into<Tag>(A, slot(j - 1), slot(j))) return e;
         for (int32_t j = len - 1; j >= 0; --j) {
-            k_adj_step<Tag><<<g, BLOCK, 0, A->stream>>>(as_cplanes<Tag>(A, slot(j)), p, (const C*)A->lam[cur],
-                                                         (C*)A->lam[cur ^ 1], A->stride, (C*)A->gF);
+            k_adj_step<Tag><<<g, b, 0, A->stream>>>((const R*)slot(j), p, (const R*)A->lam[cur],
+                                                     (R*)A->lam[cur ^ 1], (C*)A->gF);
             D3Q19_HIPCHK(hipGetLastError());
             cur ^= 1;
         }
     }
-    k_adj_u0<C><<<g, BLOCK, 0, A->stream>>>((const C*)A->lam[cur], A->stride, (const C*)A->u0, (C*)A->gu,
-                                             A->ncell);
+    k_adj_u0<Tag><<<g, b, 0, A->stream>>>((const R*)A->lam[cur], p, (const C*)A->u0, (C*)A->gu);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -322,6 +323,7 @@ int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void
     if (steps < 1) return set_error(D3Q19_EINVAL, "steps must be >= 1");
     const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (ncell * 8 >= (1ull << 32)) return set_error(D3Q19_EINVAL, "lattice too large");
+    if (cfg->nx > 65535 || cfg->ny > 65535) return set_error(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (K <= 0) K = std::max(2, (int32_t)std::lround(std::sqrt((double)steps)));
     K = std::min(K, steps);
     if (hipSetDevice(cfg->device) != hipSuccess) return set_error(D3Q19_EHIP, "hipSetDevice failed");
@@ -332,13 +334,14 @@ int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void
     A->K = K;
     A->nck = (steps + K - 1) / K;
     A->ncell = (uint32_t)ncell;
-    A->stride = ((ncell + 255) / 256) * 256 + 256;
+    A->nzp = padded_nz((uint32_t)cfg->nz);
     A->es = cfg->storage == D3Q19_F64 ? 8 : 4;
-    A->cs = A->es;
+    A->row_bytes = (size_t)Q * A->nzp * A->es;
+    A->state_bytes = (size_t)cfg->nx * cfg->ny * A->row_bytes;
     A->stream = (hipStream_t)stream;  // NULL: the default stream
-    const size_t state = (size_t)Q * A->stride * A->es;
-    const size_t lam = (size_t)Q * A->stride * A->cs;
-    const size_t vec = 3 * (size_t)ncell * A->cs;
+    const size_t state = A->state_bytes;
+    const size_t lam = A->state_bytes;  // same layout and element size
+    const size_t vec = 3 * (size_t)ncell * A->es;
     bool ok = true;
     A->ckpt.assign(A->nck, nullptr);
     A->seg.assign(std::max(2, K - 1), nullptr);

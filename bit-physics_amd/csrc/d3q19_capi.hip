@@ -1,6 +1,6 @@
 // d3q19_capi.hip -- C ABI of the D3Q19 engine (declared in include/d3q19.h).
 //
-// A lattice owns two population planes sets (ping-pong) holding the
+// A lattice owns two population buffers (ping-pong, row-blocked layout) holding the
 // post-collision deviations of the current step; the reference's
 // pre-collision state f^n is produced on demand by the read-back kernel.
 #include <hip/hip_runtime.h>
@@ -60,7 +60,9 @@ struct d3q19_lattice {
     d3q19_config cfg;
     hipStream_t stream = nullptr;  // nullptr: the default stream
     uint32_t ncell = 0;
-    size_t stride = 0;  // elements between direction planes
+    uint32_t nzp = 0;       // padded row length
+    size_t row_bytes = 0;   // 19 * nzp * esize
+    size_t lat_bytes = 0;   // one population buffer
     size_t esize = 0;
     void* buf[2] = {nullptr, nullptr};
     int cur = 0;
@@ -86,6 +88,9 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.ny = (uint32_t)L->cfg.ny;
     p.nz = (uint32_t)L->cfg.nz;
     p.ncell = L->ncell;
+    p.nzp = L->nzp;
+    p.dir_bytes = (uint32_t)(L->nzp * L->esize);
+    p.row_bytes = L->row_bytes;
     set_collision_constants(p, L->cfg.tau);
     p.force_mode = L->cfg.force_mode;
     p.wall_mask = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->cfg.wall_faces : 0u;
@@ -103,27 +108,15 @@ Params<C> make_params(const d3q19_lattice* L) {
 }
 
 template <class Tag>
-Planes<Tag> planes(const d3q19_lattice* L, int which) {
-    Planes<Tag> pl;
-    typedef typename Codec<Tag>::raw R;
-    R* base = (R*)L->buf[which];
-    for (int i = 0; i < Q; ++i) pl.p[i] = base + (size_t)i * L->stride;
-    return pl;
+typename Codec<Tag>::raw* buf_of(const d3q19_lattice* L, int which) {
+    return (typename Codec<Tag>::raw*)L->buf[which];
 }
 
-template <class Tag>
-CPlanes<Tag> cplanes(const d3q19_lattice* L, int which) {
-    CPlanes<Tag> pl;
-    Planes<Tag> q = planes<Tag>(L, which);
-    for (int i = 0; i < Q; ++i) pl.p[i] = q.p[i];
-    return pl;
-}
-
-// Launch shape of the fused step: (z, y, x) blocks of 256 threads when rows
-// are at least one wavefront long, else a linear grid over cells.
+// Launch shape of every population kernel: (z, y, x) blocks whose x extent
+// is a whole number of wavefronts of one row (64, 128 or 256 cells).
 struct StepGrid {
     dim3 grid, block;
-    bool g3;
+    uint32_t nblocks;
 };
 
 template <class Tag>
@@ -131,23 +124,17 @@ StepGrid step_grid(const d3q19_lattice* L) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
     const uint32_t block = step_block<Tag>();
-    if (nz >= 64 && ny <= 65535 && nx <= 65535) {
-        const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
-        const uint32_t by = block / bz;
-        sg.block = dim3(bz, by, 1);
-        sg.grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
-        sg.g3 = true;
-    } else {
-        sg.block = dim3(block, 1, 1);
-        sg.grid = dim3((L->ncell + block - 1) / block, 1, 1);
-        sg.g3 = false;
-    }
+    const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
+    const uint32_t by = block / bz;
+    sg.block = dim3(bz, by, 1);
+    sg.grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);
+    sg.nblocks = sg.grid.x * sg.grid.y * sg.grid.z;
     return sg;
 }
 
 template <class Tag, bool FORCED>
-void launch_k_step(d3q19_lattice* L, const StepGrid& sg, CPlanes<Tag> in, Planes<Tag> out,
-                   const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
+void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Tag>::raw* in,
+                   typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
             k_step<Tag, BC_PERIODIC, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
@@ -164,19 +151,17 @@ void launch_k_step(d3q19_lattice* L, const StepGrid& sg, CPlanes<Tag> in, Planes
 template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t g = nblocks(L->ncell);
     const StepGrid sg = step_grid<Tag>(L);
     for (int32_t s = 0; s < n; ++s) {
-        Params<C> p = make_params<C>(L);
-        p.grid3d = sg.g3 ? 1 : 0;
+        const Params<C> p = make_params<C>(L);
         const bool forced = p.force_mode != D3Q19_FORCE_NONE || p.brink != nullptr;
-        CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
-        Planes<Tag> out = planes<Tag>(L, L->cur ^ 1);
+        const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
+        typename Codec<Tag>::raw* out = buf_of<Tag>(L, L->cur ^ 1);
         float* r = (s == n - 1) ? rho : nullptr;
         float* v = (s == n - 1) ? u : nullptr;
         if (L->local) {
             // first step after a load: collide f^n in place, no streaming
-            k_collide_local<Tag><<<g, BLOCK, 0, L->stream>>>(in, out, p, r, v);
+            k_collide_local<Tag><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             L->local = false;
         } else if (forced) {
             launch_k_step<Tag, true>(L, sg, in, out, p, r, v);
@@ -194,7 +179,8 @@ template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
-    k_load<Tag, In><<<nblocks(L->ncell), BLOCK, 0, L->stream>>>(f_dev, planes<Tag>(L, L->cur), p);
+    const StepGrid sg = step_grid<Tag>(L);
+    k_load<Tag, In><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
     HIPCHK(hipGetLastError());
     L->has_state = true;
     L->local = true;
@@ -206,17 +192,17 @@ template <class Tag, class Out>
 int launch_readback(d3q19_lattice* L, Out* f, Out* rho, Out* u) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
-    const uint32_t g = nblocks(L->ncell);
-    CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
+    const StepGrid sg = step_grid<Tag>(L);
+    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_readback<Tag, BC_PERIODIC, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            k_readback<Tag, BC_PERIODIC, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
             break;
         case D3Q19_BC_YWALLS:
-            k_readback<Tag, BC_YWALLS, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            k_readback<Tag, BC_YWALLS, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
             break;
         default:
-            k_readback<Tag, BC_HALFWAY, Out><<<g, BLOCK, 0, L->stream>>>(in, p, f, rho, u);
+            k_readback<Tag, BC_HALFWAY, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
             break;
     }
     HIPCHK(hipGetLastError());
@@ -227,21 +213,21 @@ template <class Tag>
 int launch_diag(d3q19_lattice* L) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
-    const uint32_t g = nblocks(L->ncell);
-    CPlanes<Tag> in = cplanes<Tag>(L, L->cur);
+    const StepGrid sg = step_grid<Tag>(L);
+    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_diag<Tag, BC_PERIODIC><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            k_diag<Tag, BC_PERIODIC><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
         case D3Q19_BC_YWALLS:
-            k_diag<Tag, BC_YWALLS><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            k_diag<Tag, BC_YWALLS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
         default:
-            k_diag<Tag, BC_HALFWAY><<<g, BLOCK, 0, L->stream>>>(in, p, L->partials);
+            k_diag<Tag, BC_HALFWAY><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
     }
     HIPCHK(hipGetLastError());
-    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, g, NDIAG, L->dout);
+    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, sg.nblocks, NDIAG, L->dout);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -376,6 +362,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     const size_t es = storage_bytes(cfg->storage);
     if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16 or BF16");
     if (ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (ncell * 8 must fit 32 bits)");
+    if (cfg->nx > 65535 || cfg->ny > 65535) return fail(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (!(cfg->tau > 0.5)) return fail(D3Q19_EINVAL, "tau must be > 0.5");
     if (cfg->boundary < D3Q19_BC_PERIODIC || cfg->boundary > D3Q19_BC_HALFWAY)
         return fail(D3Q19_EINVAL, "bad boundary mode");
@@ -389,12 +376,15 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     L->cfg = *cfg;
     L->ncell = (uint32_t)ncell;
     L->esize = es;
-    // Plane stride: a multiple of 256 elements plus one 256-element pad, so the
-    // 19 planes do not start at the same offset modulo large powers of two.
-    L->stride = ((ncell + 255) / 256) * 256 + 256;
+    L->nzp = padded_nz((uint32_t)cfg->nz);
+    L->row_bytes = (size_t)Q * L->nzp * es;
+    L->lat_bytes = (size_t)cfg->nx * cfg->ny * L->row_bytes;
     L->stream = (hipStream_t)stream;  // NULL: the default stream
-    const size_t lat_bytes = (size_t)Q * L->stride * es;
-    const uint32_t nb = nblocks(ncell);
+    const size_t lat_bytes = L->lat_bytes;
+    // partial sums: one slot per block of the population grid (>= ncell / 256)
+    // and per block of the linear grids
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(nblocks(ncell),
+                                                     (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64));
     if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
         hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {

@@ -87,25 +87,30 @@ template <> struct Codec<TBF16> {
     }
 };
 
-// Threads per block of the fused step: one 1 KiB row segment per direction
-// and block (256 cells of 4 or 8 bytes, 512 cells of 2 bytes).
+// Threads per block of the population kernels: one 1 KiB row segment per
+// direction and block (256 cells of 4 or 8 bytes, 512 cells of 2 bytes).
 template <class Tag>
 constexpr int step_block() {
     return sizeof(typename Codec<Tag>::raw) == 2 ? 512 : 256;
 }
 
-// Per-direction base pointers: population i of cell c is base[i][c].  Each
-// base is its own plane, so offsets stay 32-bit even when the lattice is
-// larger than 4 GiB.
-template <class Tag> struct Planes {
-    typename Codec<Tag>::raw* p[Q];
-};
-template <class Tag> struct CPlanes {
-    const typename Codec<Tag>::raw* p[Q];
-};
-
+// ---------------------------------------------------------------------------
+// Population layout ("row-blocked"): the lattice is a sequence of z-rows
+// r = x * ny + y; row r holds the 19 direction segments of nzp elements each
+// (nzp = nz rounded up to 64), one after the other:
+//     byte offset of (x, y, z, i) = r * row_bytes + i * dir_bytes + z * es,
+//     dir_bytes = nzp * es, row_bytes = 19 * dir_bytes.
+// The 19 read and 19 write streams of a block then fall in a few contiguous
+// 19 * nzp * es regions instead of 38 regions a plane apart (measured on
+// MI355X: 6.26 TB/s vs 5.77 TB/s for the same 19-in/19-out stream pattern).
+// Kernels run on a (z, y, x) grid whose blocks span whole wavefronts of one
+// row, so row addresses are wavefront-uniform 64-bit scalars and per-lane
+// offsets are 32-bit.
+// ---------------------------------------------------------------------------
 template <class C> struct Params {
     uint32_t nx, ny, nz, ncell;
+    uint32_t nzp, dir_bytes;  // population layout
+    uint64_t row_bytes;
     C omega;  // 1 / tau
     C pref;   // 1 - 1 / (2 tau)   (Guo prefactor, reference/bgk.py:90)
     int32_t force_mode;
@@ -117,7 +122,6 @@ template <class C> struct Params {
     C face_u[6][3];         // half-way wall velocities
     C wtop[3], wbot[3];     // reference y-wall velocities
     int32_t local;          // state holds pre-collision f^n in place (not yet streamed)
-    int32_t grid3d;         // k_step launched on a (z, y, x) grid instead of a linear one
     // collision coefficients per weight class k (rest, face, edge), each the
     // correctly rounded value of its exact product: om w_k, 3 om w_k,
     // 4.5 om w_k, 3 pref w_k, 9 pref w_k
@@ -140,16 +144,36 @@ inline void set_collision_constants(Params<C>& p, double tau) {
     }
 }
 
-// Population i at byte offset `off` of its plane.  Offsets are 32-bit (the
-// lattice keeps ncell * 8 < 2^32), so loads address as SGPR plane base +
-// 32-bit VGPR offset, with no 64-bit arithmetic per access.
-// Pulled values are read exactly once per step: 4-byte populations load
-// with the nontemporal hint (measured faster at 256^3); 2- and 8-byte ones
-// load normally (the hint measured slower for 16-bit storage).
+// Padded row length for nz.
+__host__ __device__ inline uint32_t padded_nz(uint32_t nz) { return (nz + 63u) / 64u * 64u; }
+
+// Cell coordinates of the calling thread on the (z, y, x) grid; false for
+// padding threads.  Every wavefront lies in one row (blockDim.x is a multiple
+// of 64), so y is made an SGPR value.
+__device__ __forceinline__ bool grid_cell(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t& x,
+                                          uint32_t& y, uint32_t& z) {
+    z = blockIdx.x * blockDim.x + threadIdx.x;
+    y = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
+    x = blockIdx.z;
+    return z < nz && y < ny && x < nx;
+}
+
+__device__ __forceinline__ uint32_t block_linear_id() {
+    return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+}
+
+template <class C>
+__device__ __forceinline__ const char* row_of(const void* base, const Params<C>& p, uint32_t x, uint32_t y) {
+    return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
+}
+
+// Population i of a row at byte offset zoff.  Pulled values are read exactly
+// once per step: 4-byte populations load with the nontemporal hint (measured
+// faster at 256^3); 2- and 8-byte ones load normally (measured slower with it).
 template <class Tag>
-__device__ __forceinline__ typename Codec<Tag>::C ld(const CPlanes<Tag>& in, int i, uint32_t off) {
+__device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
     typedef typename Codec<Tag>::raw R;
-    const R* a = (const R*)((const char*)in.p[i] + off);
+    const R* a = (const R*)(row + (uint32_t)i * dir_bytes + zoff);
     if constexpr (sizeof(R) == 4)
         return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
     else
@@ -157,37 +181,51 @@ __device__ __forceinline__ typename Codec<Tag>::C ld(const CPlanes<Tag>& in, int
 }
 
 template <class Tag>
-__device__ __forceinline__ void st(const Planes<Tag>& out, int i, uint32_t off, typename Codec<Tag>::C v) {
+__device__ __forceinline__ void st(char* row, int i, uint32_t dir_bytes, uint32_t zoff, typename Codec<Tag>::C v) {
     typedef typename Codec<Tag>::raw R;
-    __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)((char*)out.p[i] + off));
+    __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)(row + (uint32_t)i * dir_bytes + zoff));
+}
+
+// In-place (unstreamed) populations of cell (x, y, z).
+template <class Tag>
+__device__ __forceinline__ void load_local(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                           uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
+    const char* row = row_of(in, p, x, y);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) d[i] = ld<Tag>(row, i, p.dir_bytes, z * ES);
+}
+
+template <class Tag>
+__device__ __forceinline__ void store_local(void* out, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                            uint32_t y, uint32_t z, const typename Codec<Tag>::C d[Q]) {
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
+    char* row = (char*)row_of(out, p, x, y);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) st<Tag>(row, i, p.dir_bytes, z * ES, d[i]);
 }
 
 // ---------------------------------------------------------------------------
 // Pull streaming with boundary handling.  Produces the pre-collision
 // deviations d_i(x) = f_i(x) - w_i of the reference state after streaming and
-// bounce-back, from the stored post-collision deviations.
+// bounce-back, from the stored post-collision deviations.  Direction i pulls
+// from row (x - cx, y - cy) at z - cz.
 // ---------------------------------------------------------------------------
 template <class Tag, int BC>
-__device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typename Codec<Tag>::C>& p,
-                                     uint32_t x, uint32_t y, uint32_t z, uint32_t cell,
-                                     typename Codec<Tag>::C d[Q]) {
+__device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                     uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     typedef typename Codec<Tag>::C C;
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
-    const uint32_t nyzb = p.ny * p.nz * ES, nzb = p.nz * ES;
-    const uint32_t xm = (x == 0 ? p.nx - 1 : x - 1) * nyzb, x0 = x * nyzb,
-                   xp = (x + 1 == p.nx ? 0 : x + 1) * nyzb;
-    const uint32_t ym = (y == 0 ? p.ny - 1 : y - 1) * nzb, y0 = y * nzb,
-                   yp = (y + 1 == p.ny ? 0 : y + 1) * nzb;
-    const uint32_t zm = (z == 0 ? p.nz - 1 : z - 1) * ES, z0 = z * ES,
-                   zp = (z + 1 == p.nz ? 0 : z + 1) * ES;
-    const uint32_t self = cell * ES;
+    const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};
+    const uint32_t ys[3] = {y == 0 ? p.ny - 1 : y - 1, y, y + 1 == p.ny ? 0 : y + 1};
+    const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1};
+    const uint32_t DB = p.dir_bytes;
+    // source index 0/1/2 along an axis for velocity component +1/0/-1
     if constexpr (BC != BC_HALFWAY) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
-            const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
-            const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
-            d[i] = ld(in, i, sx + sy + sz);
+            const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
+            d[i] = ld<Tag>(row, i, DB, zs[1 - CZ[i]] * ES);
         }
     }
     if constexpr (BC == BC_YWALLS) {
@@ -223,6 +261,7 @@ __device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typena
     }
     if constexpr (BC == BC_HALFWAY) {
         const uint32_t wm = p.wall_mask;
+        const char* self = row_of(in, p, x, y);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             int face = -1;
@@ -235,16 +274,13 @@ __device__ __forceinline__ void pull(const CPlanes<Tag>& in, const Params<typena
             if (face >= 0) {
                 C cu = C(CX[i]) * p.face_u[face][0] + C(CY[i]) * p.face_u[face][1] +
                        C(CZ[i]) * p.face_u[face][2];
-                d[i] = ld(in, OPP[i], self) + C(6) * wgt<C>(i) * cu;
+                d[i] = ld<Tag>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;
             } else {
-                const uint32_t sx = CX[i] > 0 ? xm : (CX[i] < 0 ? xp : x0);
-                const uint32_t sy = CY[i] > 0 ? ym : (CY[i] < 0 ? yp : y0);
-                const uint32_t sz = CZ[i] > 0 ? zm : (CZ[i] < 0 ? zp : z0);
-                const uint32_t src = sx + sy + sz;
-                if (p.solid && p.solid[src / ES])
-                    d[i] = ld(in, OPP[i], self);
+                const uint32_t sx = xs[1 - CX[i]], sy = ys[1 - CY[i]], sz = zs[1 - CZ[i]];
+                if (p.solid && p.solid[(sx * p.ny + sy) * p.nz + sz])
+                    d[i] = ld<Tag>(self, OPP[i], DB, z * ES);
                 else
-                    d[i] = ld(in, i, src);
+                    d[i] = ld<Tag>(row_of(in, p, sx, sy), i, DB, sz * ES);
             }
         }
     }

@@ -9,48 +9,27 @@ constexpr int BLOCK = 256;
 
 // ---------------------------------------------------------------------------
 // The hot path: one fused pull-stream / boundary / moments / BGK+Guo pass.
-// Reads the post-collision state of step n, writes that of step n+1, and
-// (optionally) rho and u of the reference state f^{n+1} from the same pass.
+// Reads the post-collision state of step n-1, writes that of step n, and
+// (optionally) rho and u of the reference state f^n from the same pass.
 // ---------------------------------------------------------------------------
 template <class Tag, int BC, bool FORCED>
 static __global__ void __launch_bounds__(step_block<Tag>())
-    k_step(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
-           float* u_out) {
+    k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
+           Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
-    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
-    uint32_t x, y, z, cell;
-    if (p.grid3d) {
-        // blocks tile (z, y) rows: z fastest, no integer division
-        z = blockIdx.x * blockDim.x + threadIdx.x;
-        y = blockIdx.y * blockDim.y + threadIdx.y;
-        x = blockIdx.z;
-        if (z >= p.nz || y >= p.ny) return;
-        cell = (x * p.ny + y) * p.nz + z;
-    } else {
-        cell = blockIdx.x * blockDim.x + threadIdx.x;
-        if (cell >= p.ncell) return;
-        cell_coords(cell, p.ny, p.nz, x, y, z);
-    }
-    const uint32_t self = cell * ES;
-    if constexpr (BC == BC_HALFWAY) {
-        if (p.solid && p.solid[cell]) {
-#pragma unroll
-            for (int i = 0; i < Q; ++i) st(out, i, self, C(0));
-            if (rho_out) rho_out[cell] = 1.0f;
-            if (u_out) {
-                u_out[cell] = 0.0f;
-                u_out[p.ncell + cell] = 0.0f;
-                u_out[2 * p.ncell + cell] = 0.0f;
-            }
-            return;
-        }
-    }
+    uint32_t x, y, z;
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
-    pull<Tag, BC>(in, p, x, y, z, cell, d);
-    C rho, ux, uy, uz;
-    collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+    C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
+    if (BC == BC_HALFWAY && p.solid && p.solid[cell]) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) st(out, i, self, d[i]);
+        for (int i = 0; i < Q; ++i) d[i] = C(0);
+    } else {
+        pull<Tag, BC>(in, p, x, y, z, d);
+        collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+    }
+    store_local<Tag>(out, p, x, y, z, d);
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;
@@ -62,40 +41,40 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 // Load a reference-layout population field f^n as the lattice state, in
 // place (pre-collision form): stored deviations d_i = f_i - w_i.
 template <class Tag, class In>
-static __global__ void __launch_bounds__(BLOCK)
-    k_load(const In* __restrict__ f, Planes<Tag> out, Params<typename Codec<Tag>::C> p) {
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_load(const In* __restrict__ f, typename Codec<Tag>::raw* __restrict__ out, Params<typename Codec<Tag>::C> p) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
+    uint32_t x, y, z;
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     const bool solid = p.solid && p.solid[cell];
+    C d[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i) {
-        const C d = solid ? C(0) : (C)((double)f[(size_t)i * p.ncell + cell] - wgt<double>(i));
-        out.p[i][cell] = Codec<Tag>::enc(d, i);
-    }
+    for (int i = 0; i < Q; ++i)
+        d[i] = solid ? C(0) : (C)((double)f[(size_t)i * p.ncell + cell] - wgt<double>(i));
+    store_local<Tag>(out, p, x, y, z, d);
 }
 
 // First step after a load: collide the in-place f^n (no streaming) into the
 // post-collision storage the fused step consumes; optionally write rho/u of f^n.
 template <class Tag>
-static __global__ void __launch_bounds__(BLOCK)
-    k_collide_local(CPlanes<Tag> in, Planes<Tag> out, Params<typename Codec<Tag>::C> p, float* rho_out,
-                    float* u_out) {
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_collide_local(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
+                    Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
+    uint32_t x, y, z;
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
     if (p.solid && p.solid[cell]) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) d[i] = Codec<Tag>::dec(in.p[i][cell], i);
+        load_local<Tag>(in, p, x, y, z, d);
         collide(d, p, cell, rho, ux, uy, uz);
     }
-#pragma unroll
-    for (int i = 0; i < Q; ++i) out.p[i][cell] = Codec<Tag>::enc(d[i], i);
+    store_local<Tag>(out, p, x, y, z, d);
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;
@@ -107,34 +86,31 @@ static __global__ void __launch_bounds__(BLOCK)
 // Pre-collision deviations of the current state at a cell: in place after a
 // load, else pull + boundary of the stored post-collision state.
 template <class Tag, int BC>
-__device__ __forceinline__ void state_at(const CPlanes<Tag>& in, const Params<typename Codec<Tag>::C>& p,
-                                         uint32_t cell, typename Codec<Tag>::C d[Q]) {
-    if (p.local) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) d[i] = Codec<Tag>::dec(in.p[i][cell], i);
-    } else {
-        uint32_t x, y, z;
-        cell_coords(cell, p.ny, p.nz, x, y, z);
-        pull<Tag, BC>(in, p, x, y, z, cell, d);
-    }
+__device__ __forceinline__ void state_at(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                         uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
+    if (p.local)
+        load_local<Tag>(in, p, x, y, z, d);
+    else
+        pull<Tag, BC>(in, p, x, y, z, d);
 }
 
-// Read-back: pull + boundary of the stored post-collision state gives the
-// reference's pre-collision f^n; write it and/or its moments.
+// Read-back: the reference's pre-collision f^n and/or its moments, in the
+// reference's (19, Nx, Ny, Nz) / (Nx, Ny, Nz) / (3, Nx, Ny, Nz) layouts.
 template <class Tag, int BC, class Out>
-static __global__ void __launch_bounds__(BLOCK)
-    k_readback(CPlanes<Tag> in, Params<typename Codec<Tag>::C> p, Out* __restrict__ f_out,
-               Out* __restrict__ rho_out, Out* __restrict__ u_out) {
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_readback(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
+               Out* __restrict__ f_out, Out* __restrict__ rho_out, Out* __restrict__ u_out) {
     typedef typename Codec<Tag>::C C;
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
-    if (cell >= p.ncell) return;
+    uint32_t x, y, z;
+    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
     const bool solid = (BC == BC_HALFWAY) && p.solid && p.solid[cell];
     if (solid) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-        state_at<Tag, BC>(in, p, cell, d);
+        state_at<Tag, BC>(in, p, x, y, z, d);
     }
     if (f_out) {
 #pragma unroll
@@ -158,15 +134,19 @@ static __global__ void __launch_bounds__(BLOCK)
 constexpr int NDIAG = 5;
 
 template <class Tag, int BC>
-static __global__ void __launch_bounds__(BLOCK)
-    k_diag(CPlanes<Tag> in, Params<typename Codec<Tag>::C> p, double* __restrict__ partials) {
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_diag(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
+           double* __restrict__ partials) {
     typedef typename Codec<Tag>::C C;
-    __shared__ double red[BLOCK / 64][NDIAG];
-    const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
+    constexpr int NW = step_block<Tag>() / 64;
+    __shared__ double red[NW][NDIAG];
     double v[NDIAG] = {0, 0, 0, 0, 0};
-    if (cell < p.ncell && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
+    uint32_t x, y, z;
+    const bool live = grid_cell(p.nx, p.ny, p.nz, x, y, z);
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
+    if (live && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
         C d[Q];
-        state_at<Tag, BC>(in, p, cell, d);
+        state_at<Tag, BC>(in, p, x, y, z, d);
         C drho, mx, my, mz, rho, ux, uy, uz;
         moments(d, drho, mx, my, mz);
         readback_moments(d, p, cell, rho, ux, uy, uz);
@@ -176,17 +156,18 @@ static __global__ void __launch_bounds__(BLOCK)
         v[3] = (double)mz;
         v[4] = 0.5 * (double)rho * ((double)ux * ux + (double)uy * uy + (double)uz * uz);
     }
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    const int lane = t & 63, wave = t >> 6;
 #pragma unroll
     for (int k = 0; k < NDIAG; ++k) {
         double s = wave_sum(v[k]);
         if (lane == 0) red[wave][k] = s;
     }
     __syncthreads();
-    if (threadIdx.x < NDIAG) {
+    if (t < NDIAG) {
         double s = 0.0;
-        for (int w = 0; w < BLOCK / 64; ++w) s += red[w][threadIdx.x];
-        partials[(size_t)blockIdx.x * NDIAG + threadIdx.x] = s;
+        for (int w = 0; w < NW; ++w) s += red[w][t];
+        partials[(size_t)block_linear_id() * NDIAG + t] = s;
     }
 }
 
