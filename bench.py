@@ -76,6 +76,207 @@ def copy_peak_gbs(amd, torch, nbytes=2 << 30, reps=20):
     return 2 * nbytes / (ms * 1e-3) / 1e9
 
 
+def adjoint_timing(torch, shape=(128, 128, 128), tau=0.6, steps=2000, K=50, storage="f32", seed=42):
+    """Forward (with checkpoints) and backward (recompute + adjoint) wall times on
+    the inverse-problem workload: 128^3 periodic, tau 0.6, 2000 steps, K = 50."""
+    from lattice_boltzmann_d3q19_amd.autograd import DifferentiableLBM
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+
+    u0 = torch.tensor(random_solenoidal(shape, 0.02, seed=seed), device="cuda", dtype=torch.float32)
+    F = torch.zeros_like(u0)
+    g = torch.tensor(random_solenoidal(shape, 1e-3, seed=seed + 1), device="cuda", dtype=torch.float32)
+    solver = DifferentiableLBM(shape, tau, steps, checkpoint_every=K, storage=storage)
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    solver.forward_raw(u0, F)  # warm-up
+    solver.backward_raw(g)
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    solver.forward_raw(u0, F)
+    ev[1].record(stream)
+    solver.backward_raw(g)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    fwd = ev[0].elapsed_time(ev[1])
+    bwd = ev[1].elapsed_time(ev[2])
+    mem = solver.memory_bytes
+    solver.close()
+    return {
+        "workload": f"inverse-{shape[0]}^3-periodic-{steps}steps-K{K}-{storage}",
+        "forward_ms": round(fwd, 3),
+        "backward_ms": round(bwd, 3),
+        "adjoint_forward_ratio": round(bwd / fwd, 4),
+        "forward_GLUPS": round(shape[0] * shape[1] * shape[2] * steps / (fwd * 1e-3) / 1e9, 3),
+        "checkpoint_memory_GB": round(mem / 1e9, 3),
+    }
+
+
+def study_turbulence(amd, torch, n=384, steps=4000, every=250, tau=0.51, rms=0.03, seed=42):
+    """Decaying turbulence: FP16 / BF16 (f - w) and FP32 storage against FP64 storage."""
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal, spectral_divergence
+
+    shape = (n, n, n)
+    u0 = random_solenoidal(shape, rms, seed=seed)
+    div = spectral_divergence(u0)
+    runs = {}
+    for storage in ("f64", "f32", "f16", "bf16"):
+        lat = amd.Lattice(shape, tau, storage=storage)
+        lat.set_equilibrium(None, u0)
+        series = []
+        total_ms = 0.0
+        stream = torch.cuda.current_stream()
+        for k in range(0, steps + 1, every):
+            if k > 0:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                lat.step(every)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                total_ms += e0.elapsed_time(e1)
+            d = lat.diagnostics()
+            series.append((k, d["kinetic_energy"], lat.enstrophy(), d["mass"]))
+        lat.close()
+        runs[storage] = {"series": series, "GLUPS": n ** 3 * steps / (total_ms * 1e-3) / 1e9}
+        print(f"[study] turbulence {storage}: {runs[storage]['GLUPS']:.2f} GLUPS, "
+              f"KE {series[0][1]:.6e} -> {series[-1][1]:.6e}", file=sys.stderr, flush=True)
+    ref = runs["f64"]["series"]
+    out = {"study": "turbulence", "workload": f"decaying-turbulence-{n}^3-periodic-tau{tau}-{steps}steps",
+           "ic": {"rms": rms, "spectrum": "k^4 exp(-2 (k/8)^2)", "seed": seed, "spectral_divergence": div},
+           "reference": "FP64 storage (FP64 collision) on the GPU"}
+    for storage in ("f32", "f16", "bf16"):
+        ser = runs[storage]["series"]
+        ke = max(abs(a[1] - b[1]) / b[1] for a, b in zip(ser, ref))
+        en = max(abs(a[2] - b[2]) / b[2] for a, b in zip(ser, ref))
+        ms = max(abs(a[3] - b[3]) / b[3] for a, b in zip(ser, ref))
+        out[storage] = {"GLUPS": round(runs[storage]["GLUPS"], 3), "ke_max_rel_err": ke,
+                        "enstrophy_max_rel_err": en, "mass_max_rel_err": ms,
+                        "ke_final_rel_err": abs(ser[-1][1] - ref[-1][1]) / ref[-1][1]}
+    out["f64"] = {"GLUPS": round(runs["f64"]["GLUPS"], 3),
+                  "ke": [r[1] for r in ref], "enstrophy": [r[2] for r in ref], "mass": [r[3] for r in ref],
+                  "steps": [r[0] for r in ref]}
+    out["fp16_over_fp32_GLUPS"] = round(runs["f16"]["GLUPS"] / runs["f32"]["GLUPS"], 4)
+    return out
+
+
+def study_inverse(torch, n=128, steps=2000, K=50, tau=0.6, ndir=16, eps=1e-3, seed=42):
+    """Inverse problem: adjoint gradients (FP32 and FP64) against central
+    differences of the FP64 forward map along seeded random unit directions
+    in (u0, F) (F components drawn 1e-3 smaller), step eps."""
+    from lattice_boltzmann_d3q19_amd.autograd import DifferentiableLBM, velocity_matching_loss
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+
+    shape = (n, n, n)
+    dev = "cuda"
+    u_true = torch.tensor(random_solenoidal(shape, 0.02, seed=seed), device=dev)
+    zeros = torch.zeros_like(u_true)
+    f64 = DifferentiableLBM(shape, tau, steps, checkpoint_every=K, storage="f64")
+    target = f64.forward_raw(u_true, zeros).clone()
+    u0 = torch.tensor(random_solenoidal(shape, 0.02, seed=seed + 7), device=dev)
+
+    def grad(storage):
+        s = f64 if storage == "f64" else DifferentiableLBM(shape, tau, steps, checkpoint_every=K, storage=storage)
+        a = u0.clone().requires_grad_(True)
+        b = zeros.clone().requires_grad_(True)
+        loss = velocity_matching_loss(s(a, b), target)
+        loss.backward()
+        if s is not f64:
+            s.close()
+        return float(loss), a.grad.double(), b.grad.double()
+
+    L32, gu32, gF32 = grad("f32")
+    L64, gu64, gF64 = grad("f64")
+
+    def loss_at(a, b):
+        return float(velocity_matching_loss(f64.forward_raw(a, b), target))
+
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    fds, ad32, ad64 = [], [], []
+    for k in range(ndir):
+        du = torch.randn(u0.shape, generator=gen, dtype=torch.float64).to(dev)
+        dF = 1e-3 * torch.randn(u0.shape, generator=gen, dtype=torch.float64).to(dev)
+        nrm = (du.square().sum() + dF.square().sum()).sqrt()  # unit direction in (u0, F)
+        du, dF = du / nrm, dF / nrm
+        fds.append((loss_at(u0 + eps * du, zeros + eps * dF) - loss_at(u0 - eps * du, zeros - eps * dF)) / (2 * eps))
+        ad32.append(float((gu32 * du).sum() + (gF32 * dF).sum()))
+        ad64.append(float((gu64 * du).sum() + (gF64 * dF).sum()))
+    scale = max(abs(v) for v in fds)
+    rel = lambda a, b: float((a - b).norm() / b.norm())
+    out = {"study": "inverse", "workload": f"inverse-{n}^3-periodic-tau{tau}-{steps}steps-K{K}",
+           "loss_f32": L32, "loss_f64": L64, "fd_eps": eps, "directions": ndir,
+           "fd_max_err_f32_adjoint": max(abs(a - f) for a, f in zip(ad32, fds)) / scale,
+           "fd_max_err_f64_adjoint": max(abs(a - f) for a, f in zip(ad64, fds)) / scale,
+           "f32_vs_f64_adjoint_grad_u0_rel": rel(gu32, gu64), "f32_vs_f64_adjoint_grad_F_rel": rel(gF32, gF64),
+           "fd": fds, "adjoint_f32": ad32}
+    out["timing"] = adjoint_timing(torch, shape, tau, steps, K, "f32", seed)
+    f64.close()
+    return out
+
+
+def study_channel(amd, torch, shape=(256, 128, 128), tau=0.6, steps=4000, gx=1e-6, budget=64e9,
+                  ndir=4, eps=1e-3, seed=42):
+    """Channel with Brinkman obstacles: dJ/ds for J = mean outlet x-velocity,
+    FP32 compute with FP16 (f - w) checkpoints vs FP32 checkpoints (K
+    autotuned to the HBM budget), central differences of the FP64 forward."""
+    from lattice_boltzmann_d3q19_amd.autograd import DifferentiableLBM, outlet_velocity
+    from lattice_boltzmann_d3q19_amd.initial import random_spheres
+
+    s_np = random_spheres(shape, 6.0, 12.0, 0.05, seed=seed, wall_axis=1)
+    dev = "cuda"
+    s = torch.tensor(s_np, device=dev)
+    u0 = torch.zeros((3,) + shape, device=dev, dtype=torch.float64)
+    F = torch.zeros_like(u0)
+    F[0] = gx
+    kw = dict(boundary="halfway", wall_faces=("-y", "+y"), brinkman=True, checkpoint_every=0,
+              memory_budget=int(budget))
+    out = {"study": "channel", "workload": f"channel-{shape[0]}x{shape[1]}x{shape[2]}-tau{tau}-{steps}steps",
+           "solid_volume_fraction": float(s_np.mean()), "gx": gx, "memory_budget_GB": budget / 1e9}
+    grads = {}
+    for cks in ("f16", "f32"):
+        solver = DifferentiableLBM(shape, tau, steps, storage="f32", checkpoint_storage=cks, **kw)
+        stream = torch.cuda.current_stream()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev[0].record(stream)
+        uT = solver.forward_raw(u0, F, s)
+        ev[1].record(stream)
+        g = torch.zeros_like(uT)
+        g[0, -1] = 1.0 / (shape[1] * shape[2])
+        _, _, gs = solver.backward_raw(g)
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        fwd, bwd = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+        grads[cks] = gs.double()
+        out[f"checkpoints_{cks}"] = {"K": solver.checkpoint_every, "memory_GB": round(solver.memory_bytes / 1e9, 3),
+                                     "J": float(outlet_velocity(uT)), "forward_ms": round(fwd, 1),
+                                     "backward_ms": round(bwd, 1), "adjoint_forward_ratio": round(bwd / fwd, 3)}
+        solver.close()
+        torch.cuda.empty_cache()
+        print(f"[study] channel {cks} checkpoints: {out[f'checkpoints_{cks}']}", file=sys.stderr, flush=True)
+    a, b = grads["f16"], grads["f32"]
+    out["grad_rel_err_f16_vs_f32_checkpoints"] = float((a - b).norm() / b.norm())
+
+    # central differences of J(s) with the FP64 forward (stateful lattice)
+    def J(sv):
+        lat = amd.Lattice(shape, tau, storage="f64", boundary="halfway", wall_faces=("-y", "+y"),
+                          force=(gx, 0.0, 0.0))
+        lat.set_brinkman(sv).set_equilibrium()
+        lat.step(steps)
+        _, u = lat.moments()
+        lat.close()
+        return float(u[0, -1].mean())
+
+    gen = torch.Generator(device="cpu").manual_seed(seed)
+    fds, ads = [], []
+    for _ in range(ndir):
+        d = torch.randn(shape, generator=gen, dtype=torch.float64)
+        d /= d.norm()  # unit direction: eps is the step length in parameter space
+        fds.append((J((s.cpu() + eps * d).numpy()) - J((s.cpu() - eps * d).numpy())) / (2 * eps))
+        ads.append(float((b.cpu() * d).sum()))
+    out["fd"] = fds
+    out["adjoint_f32_checkpoints"] = ads
+    out["fd_max_rel_err"] = max(abs(x - y) for x, y in zip(ads, fds)) / max(abs(v) for v in fds)
+    return out
+
+
 def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     """The FP64 CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
     import numpy as np
@@ -109,6 +310,9 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--study", choices=("turbulence", "inverse", "channel"), default=None,
+                    help="run a validation study instead of the headline benchmark")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this engine has no multi-GPU path "
@@ -122,6 +326,15 @@ def main() -> int:
     from lattice_boltzmann_d3q19_amd.initial import taylor_green, taylor_green_ke_decay
 
     torch.cuda.set_device(0)
+    if args.study == "turbulence":
+        print(json.dumps(study_turbulence(amd, torch)), flush=True)
+        return 0
+    if args.study == "inverse":
+        print(json.dumps(study_inverse(torch)), flush=True)
+        return 0
+    if args.study == "channel":
+        print(json.dumps(study_channel(amd, torch)), flush=True)
+        return 0
     n = args.size
     shape = (n, n, n)
     ncell = n ** 3
@@ -192,6 +405,10 @@ def main() -> int:
             "frac": round(ach16 / HBM_PEAK_GBS, 4),
             "ke_rel_diff_vs_fp32": abs(diag16["kinetic_energy"] - diag["kinetic_energy"]) / diag["kinetic_energy"],
         }
+
+    if not args.no_adjoint:
+        result["adjoint"] = adjoint_timing(torch)
+        result["adjoint_forward_ratio"] = result["adjoint"]["adjoint_forward_ratio"]
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.tau, rho0, u0, args.cpu_steps)

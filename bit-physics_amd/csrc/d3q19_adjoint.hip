@@ -1,17 +1,20 @@
-// d3q19_adjoint.hip -- discrete adjoint of the periodic Guo-forced D3Q19 step.
+// d3q19_adjoint.hip -- discrete adjoint of the D3Q19 BGK + Guo (+ Brinkman) step.
 //
-// Forward map (periodic, per-cell force F):
-//   f^0 = feq(1, u0),  f^{n+1} = S C(f^n; F),  u^T = (m(f^T) + F/2) / rho(f^T)
-// with C the BGK+Guo collision and S the streaming permutation.  The engine
-// stores post-collision states, so the forward runs on
+// Forward map (per-cell force F, optional Brinkman solid fraction s):
+//   f^0 = feq(1, u0),  f^{n+1} = S C(f^n; F - s m),  u^T = (m + (F - s m)/2) / rho  at f^T
+// with C the BGK+Guo collision and S the streaming permutation (periodic, or
+// half-way bounce-back on stationary wall faces -- also a permutation).  The
+// engine stores post-collision states, so the forward runs on
 //   H_0 = S^-1 f^0,  H_{j+1} = k_step(H_j),  f^j = S H_j.
 // Backward (lambda^n = dL/df^n):
 //   lambda^T   = J_u(f^T)^T dL/du^T
-//   lambda^n   = J_C(f^n)^T S^T lambda^{n+1},   (S^T mu)_i(x) = mu_i(x + c_i)
-//   dL/dF     += dC/dF(f^n)^T S^T lambda^{n+1}  +  dL/du^T * 1/(2 rho^T)
+//   lambda^n   = J_C(f^n)^T S^T lambda^{n+1},
+//                (S^T mu)_i(x) = mu_i(x + c_i), or mu_opp(i)(x) when x + c_i is across a wall
+//   dL/dF, dL/ds accumulate the parameter parts of each step's Jacobian
 //   dL/du0     = dfeq(1,u0)/du0^T lambda^0
 // States between checkpoints (every K steps) are recomputed by the forward
-// kernel itself, writing straight into the segment slots.
+// kernel itself, writing straight into the segment slots.  Checkpoints may be
+// stored as FP16 (f - w) and expanded to the working precision on use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,9 +31,26 @@ using namespace d3q19;
 
 namespace {
 
-// Initial post-collision "pre-image": H_0 = S^-1 feq(1, u0), i.e.
-// H_0,i(y) = feq_i(1, u0(y + c_i)) - w_i.
-template <class Tag>
+// True when the pull source x - c of a cell crosses a wall face (half-way mode).
+template <class C>
+__device__ __forceinline__ bool crosses_wall(int cx, int cy, int cz, uint32_t x, uint32_t y, uint32_t z,
+                                             const Params<C>& p) {
+    const uint32_t wm = p.wall_mask;
+    return (cx > 0 && x == 0 && (wm & 1u)) || (cx < 0 && x + 1 == p.nx && (wm & 2u)) ||
+           (cy > 0 && y == 0 && (wm & 4u)) || (cy < 0 && y + 1 == p.ny && (wm & 8u)) ||
+           (cz > 0 && z == 0 && (wm & 16u)) || (cz < 0 && z + 1 == p.nz && (wm & 32u));
+}
+
+template <class C>
+__device__ __forceinline__ C feq_dev(int i, const C* u0, uint32_t N, uint32_t c) {
+    const C ux = u0[c], uy = u0[N + c], uz = u0[2 * N + c];
+    const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
+    return wgt<C>(i) * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * (ux * ux + uy * uy + uz * uz));
+}
+
+// Initial post-collision pre-image H_0 = S^-1 feq(1, u0):
+//   H_0,i(x) = feq_i(1, u0(x + c_i)) - w_i, or feq_opp(i)(1, u0(x)) - w when x + c_i is across a wall.
+template <class Tag, int BC>
 __global__ void __launch_bounds__(step_block<Tag>())
     k_adj_unstream(const typename Codec<Tag>::C* __restrict__ u0, typename Codec<Tag>::raw* __restrict__ out,
                    Params<typename Codec<Tag>::C> p) {
@@ -38,75 +58,90 @@ __global__ void __launch_bounds__(step_block<Tag>())
     uint32_t x, y, z;
     if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
     const uint32_t N = p.ncell;
+    const uint32_t self = (x * p.ny + y) * p.nz + z;
     C d[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        const uint32_t sx = (x + p.nx + CX[i]) % p.nx, sy = (y + p.ny + CY[i]) % p.ny, sz = (z + p.nz + CZ[i]) % p.nz;
-        const uint32_t src = (sx * p.ny + sy) * p.nz + sz;
-        const C ux = u0[src], uy = u0[N + src], uz = u0[2 * N + src];
-        const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
-        d[i] = wgt<C>(i) * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * (ux * ux + uy * uy + uz * uz));
+        if (BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p)) {
+            d[i] = feq_dev<C>(OPP[i], u0, N, self);
+        } else {
+            const uint32_t sx = (x + p.nx + CX[i]) % p.nx, sy = (y + p.ny + CY[i]) % p.ny,
+                           sz = (z + p.nz + CZ[i]) % p.nz;
+            d[i] = feq_dev<C>(i, u0, N, (sx * p.ny + sy) * p.nz + sz);
+        }
     }
     store_local<Tag>(out, p, x, y, z, d);
 }
 
-// Seed: lambda^T_j = sum_a g_a (c_ja - u_a) / rho ;  gF_a = g_a / (2 rho).
-template <class Tag>
+// Seed at f^T: u = ((1 - s/2) m + F/2)/rho
+//   lambda^T_j = (g.c_j (1 - s/2) - g.u) / rho ;  gF = g / (2 rho) ;  gs = -(g.m) / (2 rho)
+template <class Tag, int BC, bool BRINK>
 __global__ void __launch_bounds__(step_block<Tag>())
     k_adj_seed(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
                const typename Codec<Tag>::C* __restrict__ g, typename Codec<Tag>::raw* __restrict__ lam,
-               typename Codec<Tag>::C* __restrict__ gF) {
+               typename Codec<Tag>::C* __restrict__ gF, typename Codec<Tag>::C* __restrict__ gs) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
     if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
-    pull<Tag, BC_PERIODIC>(H, p, x, y, z, d);
+    pull<Tag, BC>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
     const C ir = C(1) / rho;
     const uint32_t N = p.ncell;
-    const C fx = p.force[cell], fy = p.force[N + cell], fz = p.force[2 * N + cell];
+    const C s = BRINK ? p.brink[cell] : C(0);
+    const C fx = p.force[cell] - s * mx, fy = p.force[N + cell] - s * my, fz = p.force[2 * N + cell] - s * mz;
     const C ux = (mx + C(0.5) * fx) * ir, uy = (my + C(0.5) * fy) * ir, uz = (mz + C(0.5) * fz) * ir;
     const C gx = g[cell], gy = g[N + cell], gz = g[2 * N + cell];
     const C gu = gx * ux + gy * uy + gz * uz;
+    const C keep = C(1) - C(0.5) * s;
     C l[Q];
 #pragma unroll
-    for (int j = 0; j < Q; ++j) l[j] = (C(CX[j]) * gx + C(CY[j]) * gy + C(CZ[j]) * gz - gu) * ir;
+    for (int j = 0; j < Q; ++j) l[j] = ((C(CX[j]) * gx + C(CY[j]) * gy + C(CZ[j]) * gz) * keep - gu) * ir;
     store_local<Tag>(lam, p, x, y, z, l);
     gF[cell] = C(0.5) * gx * ir;
     gF[N + cell] = C(0.5) * gy * ir;
     gF[2 * N + cell] = C(0.5) * gz * ir;
+    if (BRINK) gs[cell] = -C(0.5) * (gx * mx + gy * my + gz * mz) * ir;
 }
 
-// One reverse step: lambda_out = J_C(f^n)^T S^T lambda_in, gF += dC/dF^T S^T lambda_in.
-template <class Tag>
+// One reverse step: lambda_out = J_C(f^n)^T S^T lambda_in; the parameter
+// parts go to gF (force) and gs (Brinkman fraction).
+template <class Tag, int BC, bool BRINK>
 __global__ void __launch_bounds__(step_block<Tag>())
     k_adj_step(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
                const typename Codec<Tag>::raw* __restrict__ lam_in, typename Codec<Tag>::raw* __restrict__ lam_out,
-               typename Codec<Tag>::C* __restrict__ gF) {
+               typename Codec<Tag>::C* __restrict__ gF, typename Codec<Tag>::C* __restrict__ gs) {
     typedef typename Codec<Tag>::C C;
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     uint32_t x, y, z;
     if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     const uint32_t N = p.ncell;
-    // S^T: mu_i(x) = lambda_i(x + c_i)  (the pull run backwards)
+    // S^T: mu_i(x) = lambda_i(x + c_i)  (the pull run backwards; bounce links reversed)
     const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};
     const uint32_t ys[3] = {y == 0 ? p.ny - 1 : y - 1, y, y + 1 == p.ny ? 0 : y + 1};
     const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1};
+    const char* self_row = row_of(lam_in, p, x, y);
     C mu[Q];
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-        mu[i] = ld<Tag>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
+    for (int i = 0; i < Q; ++i) {
+        if (BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p))
+            mu[i] = ld<Tag>(self_row, OPP[i], p.dir_bytes, z * ES);
+        else
+            mu[i] = ld<Tag>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
+    }
     C d[Q];
-    pull<Tag, BC_PERIODIC>(H, p, x, y, z, d);
+    pull<Tag, BC>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
     const C ir = C(1) / rho;
-    const C F[3] = {p.force[cell], p.force[N + cell], p.force[2 * N + cell]};
+    const C s = BRINK ? p.brink[cell] : C(0);
+    const C m[3] = {mx, my, mz};
+    const C F[3] = {p.force[cell] - s * mx, p.force[N + cell] - s * my, p.force[2 * N + cell] - s * mz};
     const C u[3] = {(mx + C(0.5) * F[0]) * ir, (my + C(0.5) * F[1]) * ir, (mz + C(0.5) * F[2]) * ir};
     const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const C om = p.omega, pref = p.pref;
@@ -116,29 +151,35 @@ __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const C c[3] = {C(CX[i]), C(CY[i]), C(CZ[i])};
-        const C s = mu[i] * wgt<C>(i);
+        const C sw = mu[i] * wgt<C>(i);
         const C cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
         const C cF = c[0] * F[0] + c[1] * F[1] + c[2] * F[2];
-        a_rho += s * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
+        a_rho += sw * (C(1) + C(3) * cu + C(4.5) * cu * cu - C(1.5) * usq);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             // d(feq_i/tau)/du_a = w rho (3 c_a + 9 cu c_a - 3 u_a)/tau ; dG/du_a = pref w (-3 F_a + 9 c_a cF)
-            b[a] += s * (om * rho * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]) +
-                         pref * (C(9) * c[a] * cF - C(3) * F[a]));
+            b[a] += sw * (om * rho * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]) +
+                          pref * (C(9) * c[a] * cF - C(3) * F[a]));
             // explicit dG/dF_a = pref w (3 (c_a - u_a) + 9 cu c_a)
-            ge[a] += s * pref * (C(3) * (c[a] - u[a]) + C(9) * cu * c[a]);
+            ge[a] += sw * pref * (C(3) * (c[a] - u[a]) + C(9) * cu * c[a]);
         }
     }
     a_rho *= om;
+    // total derivative through the effective force (explicit + via u = (m + F/2)/rho)
+    const C gt[3] = {ge[0] + C(0.5) * b[0] * ir, ge[1] + C(0.5) * b[1] * ir, ge[2] + C(0.5) * b[2] * ir};
     const C keep = C(1) - om;
     C l[Q];
 #pragma unroll
-    for (int j = 0; j < Q; ++j)
-        l[j] = keep * mu[j] + a_rho +
-               (b[0] * (C(CX[j]) - u[0]) + b[1] * (C(CY[j]) - u[1]) + b[2] * (C(CZ[j]) - u[2])) * ir;
+    for (int j = 0; j < Q; ++j) {
+        C v = keep * mu[j] + a_rho +
+              (b[0] * (C(CX[j]) - u[0]) + b[1] * (C(CY[j]) - u[1]) + b[2] * (C(CZ[j]) - u[2])) * ir;
+        if (BRINK) v -= s * (C(CX[j]) * gt[0] + C(CY[j]) * gt[1] + C(CZ[j]) * gt[2]);  // F = F0 - s m
+        l[j] = v;
+    }
     store_local<Tag>(lam_out, p, x, y, z, l);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) gF[(size_t)a * N + cell] += ge[a] + C(0.5) * b[a] * ir;
+    for (int a = 0; a < 3; ++a) gF[(size_t)a * N + cell] += gt[a];
+    if (BRINK) gs[cell] -= gt[0] * m[0] + gt[1] * m[1] + gt[2] * m[2];
 }
 
 // dL/du0_a = sum_i lambda^0_i w_i (3 c_a + 9 cu c_a - 3 u_a)   (rho = 1)
@@ -158,32 +199,51 @@ __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const C c[3] = {C(CX[i]), C(CY[i]), C(CZ[i])};
-        const C s = l[i] * wgt<C>(i);
+        const C sw = l[i] * wgt<C>(i);
         const C cu = c[0] * u[0] + c[1] * u[1] + c[2] * u[2];
 #pragma unroll
-        for (int a = 0; a < 3; ++a) g[a] += s * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]);
+        for (int a = 0; a < 3; ++a) g[a] += sw * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]);
     }
     gu0[cell] = g[0];
     gu0[N + cell] = g[1];
     gu0[2 * N + cell] = g[2];
 }
 
+// Re-encode a state between storage formats (checkpoint compression).
+template <class TIn, class TOut>
+__global__ void __launch_bounds__(256)
+    k_convert_state(const typename Codec<TIn>::raw* __restrict__ in, Params<typename Codec<TIn>::C> pin,
+                    typename Codec<TOut>::raw* __restrict__ out, Params<typename Codec<TOut>::C> pout) {
+    uint32_t x, y, z;
+    if (!grid_cell(pin.nx, pin.ny, pin.nz, x, y, z)) return;
+    typename Codec<TIn>::C a[Q];
+    typename Codec<TOut>::C b[Q];
+    load_local<TIn>(in, pin, x, y, z, a);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) b[i] = (typename Codec<TOut>::C)a[i];
+    store_local<TOut>(out, pout, x, y, z, b);
+}
+
 }  // namespace
 
 struct d3q19_adjoint {
     d3q19_config cfg;
+    d3q19_adjoint_options opt;
     int32_t T = 0, K = 0, nck = 0;
+    bool ckpt_same = true;  // checkpoints in the working storage
+    bool brink = false;
     hipStream_t stream = nullptr;  // nullptr: the default stream
-    uint32_t ncell = 0;
-    size_t es = 0, state_bytes = 0, row_bytes = 0;
-    uint32_t nzp = 0;
+    uint32_t ncell = 0, nzp = 0;
+    size_t es = 0, state_bytes = 0, ckpt_bytes = 0;
     std::vector<void*> ckpt;  // H_{kK}
-    std::vector<void*> seg;   // recompute slots (>= 2, also the forward ping-pong)
+    std::vector<void*> seg;   // working states: recompute slots, also the forward ping-pong
     void* lam[2] = {nullptr, nullptr};
     void* u0 = nullptr;
     void* force = nullptr;
+    void* s = nullptr;    // Brinkman fraction
     void* gF = nullptr;
-    void* gu = nullptr;    // staged dL/du^T, reused for dL/du0
+    void* gu = nullptr;   // staged dL/du^T, reused for dL/du0
+    void* gs = nullptr;   // dL/ds
     void* Hfinal = nullptr;
     bool have_forward = false;
     size_t bytes = 0;
@@ -194,7 +254,7 @@ namespace {
 bool ct_double(const d3q19_adjoint* A) { return A->cfg.storage == D3Q19_F64; }
 
 template <class C>
-Params<C> adj_params(const d3q19_adjoint* A) {
+Params<C> adj_params(const d3q19_adjoint* A, size_t es) {
     Params<C> p;
     std::memset(&p, 0, sizeof(p));
     p.nx = A->cfg.nx;
@@ -202,20 +262,22 @@ Params<C> adj_params(const d3q19_adjoint* A) {
     p.nz = A->cfg.nz;
     p.ncell = A->ncell;
     p.nzp = A->nzp;
-    p.dir_bytes = (uint32_t)(A->nzp * A->es);
-    p.row_bytes = A->row_bytes;
+    p.dir_bytes = (uint32_t)(A->nzp * es);
+    p.row_bytes = (uint64_t)Q * A->nzp * es;
     set_collision_constants(p, A->cfg.tau);
     p.force_mode = FORCE_FIELD;
     p.force = (const C*)A->force;
+    p.brink = A->brink ? (const C*)A->s : nullptr;
+    p.wall_mask = A->cfg.boundary == D3Q19_BC_HALFWAY ? A->cfg.wall_faces : 0u;
     return p;
 }
 
 // (z, y, x) launch shape shared by every kernel here.
 template <class Tag>
-void adj_grid(const d3q19_adjoint* A, dim3& grid, dim3& block) {
+void adj_grid(const d3q19_adjoint* A, dim3& grid, dim3& block, uint32_t threads = step_block<Tag>()) {
     const uint32_t nz = A->cfg.nz;
-    const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
-    const uint32_t by = step_block<Tag>() / bz;
+    const uint32_t bz = row_block(nz);
+    const uint32_t by = threads / bz;
     block = dim3(bz, by, 1);
     grid = dim3((nz + bz - 1) / bz, (A->cfg.ny + by - 1) / by, A->cfg.nx);
 }
@@ -224,10 +286,31 @@ template <class Tag>
 int step_into(d3q19_adjoint* A, void* in, void* out) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
-    const Params<C> p = adj_params<C>(A);
+    const Params<C> p = adj_params<C>(A, A->es);
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
-    k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
+    if (A->cfg.boundary == D3Q19_BC_HALFWAY)
+        k_step<Tag, BC_HALFWAY, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
+    else
+        k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
+    D3Q19_HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+// Working state <-> FP16 checkpoint.
+template <class Tag>
+int convert(d3q19_adjoint* A, const void* in, void* out, bool to_ckpt) {
+    typedef typename Codec<Tag>::C C;
+    dim3 g, b;
+    adj_grid<Tag>(A, g, b, 256);
+    const Params<C> pw = adj_params<C>(A, A->es);
+    const Params<float> ph = adj_params<float>(A, 2);
+    if (to_ckpt)
+        k_convert_state<Tag, TF16><<<g, b, 0, A->stream>>>((const typename Codec<Tag>::raw*)in, pw,
+                                                           (uint16_t*)out, ph);
+    else
+        k_convert_state<TF16, Tag><<<g, b, 0, A->stream>>>((const uint16_t*)in, ph,
+                                                           (typename Codec<Tag>::raw*)out, pw);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -236,57 +319,76 @@ template <class Tag>
 int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
-    const Params<C> p = adj_params<C>(A);
+    const Params<C> p = adj_params<C>(A, A->es);
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
-    k_adj_unstream<Tag><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)A->ckpt[0], p);
+    void* h0 = A->ckpt_same ? A->ckpt[0] : A->seg[0];
+    if (A->cfg.boundary == D3Q19_BC_HALFWAY)
+        k_adj_unstream<Tag, BC_HALFWAY><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)h0, p);
+    else
+        k_adj_unstream<Tag, BC_PERIODIC><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)h0, p);
     D3Q19_HIPCHK(hipGetLastError());
-    void* cur = A->ckpt[0];
-    int pp = 0;  // next ping-pong slot
+    if (!A->ckpt_same)
+        if (int e = convert<Tag>(A, h0, A->ckpt[0], true)) return e;
+    void* cur = h0;
     for (int32_t j = 0; j < A->T; ++j) {
         const int32_t nxt = j + 1;
+        const bool is_ck = (nxt % A->K == 0) && (nxt / A->K < A->nck);
         void* out;
-        if (nxt % A->K == 0 && nxt / A->K < A->nck) {
+        if (is_ck && A->ckpt_same)
             out = A->ckpt[nxt / A->K];
-        } else {
-            out = A->seg[pp];
-            if (out == cur) out = A->seg[pp ^ 1];
-            pp = (out == A->seg[0]) ? 1 : 0;
-        }
+        else
+            out = (cur == A->seg[0]) ? A->seg[1] : A->seg[0];
         if (int e = step_into<Tag>(A, cur, out)) return e;
+        if (is_ck && !A->ckpt_same)
+            if (int e = convert<Tag>(A, out, A->ckpt[nxt / A->K], true)) return e;
         cur = out;
     }
     A->Hfinal = cur;
-    if (out_dt == D3Q19_F64)
-        k_readback<Tag, BC_PERIODIC, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                     (double*)u_out_dev);
-    else
-        k_readback<Tag, BC_PERIODIC, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                    (float*)u_out_dev);
+    if (A->cfg.boundary == D3Q19_BC_HALFWAY) {
+        if (out_dt == D3Q19_F64)
+            k_readback<Tag, BC_HALFWAY, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                        (double*)u_out_dev);
+        else
+            k_readback<Tag, BC_HALFWAY, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                       (float*)u_out_dev);
+    } else {
+        if (out_dt == D3Q19_F64)
+            k_readback<Tag, BC_PERIODIC, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                         (double*)u_out_dev);
+        else
+            k_readback<Tag, BC_PERIODIC, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
+                                                                        (float*)u_out_dev);
+    }
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
 
-template <class Tag>
-int run_backward(d3q19_adjoint* A) {
+template <class Tag, int BC, bool BRINK>
+int run_backward_t(d3q19_adjoint* A) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
-    const Params<C> p = adj_params<C>(A);
+    const Params<C> p = adj_params<C>(A, A->es);
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
-    k_adj_seed<Tag><<<g, b, 0, A->stream>>>((const R*)A->Hfinal, p, (const C*)A->gu, (R*)A->lam[0], (C*)A->gF);
+    k_adj_seed<Tag, BC, BRINK><<<g, b, 0, A->stream>>>((const R*)A->Hfinal, p, (const C*)A->gu, (R*)A->lam[0],
+                                                        (C*)A->gF, (C*)A->gs);
     D3Q19_HIPCHK(hipGetLastError());
     int cur = 0;
     for (int32_t k = A->nck - 1; k >= 0; --k) {
         const int32_t n0 = k * A->K;
         const int32_t len = std::min(A->K, A->T - n0);
-        // slot 0 = checkpoint, slots 1..len-1 recomputed into seg[0..len-2]
-        auto slot = [&](int32_t j) -> void* { return j == 0 ? A->ckpt[k] : A->seg[j - 1]; };
+        // slot j holds H_{n0 + j}: the checkpoint itself (same storage) or its
+        // expansion into seg[0]; the rest recomputed into seg[]
+        const int off = A->ckpt_same ? 1 : 0;
+        auto slot = [&](int32_t j) -> void* { return (j == 0 && A->ckpt_same) ? A->ckpt[k] : A->seg[j - off]; };
+        if (!A->ckpt_same)
+            if (int e = convert<Tag>(A, A->ckpt[k], A->seg[0], false)) return e;
         for (int32_t j = 1; j < len; ++j)
             if (int e = step_into<Tag>(A, slot(j - 1), slot(j))) return e;
         for (int32_t j = len - 1; j >= 0; --j) {
-            k_adj_step<Tag><<<g, b, 0, A->stream>>>((const R*)slot(j), p, (const R*)A->lam[cur],
-                                                     (R*)A->lam[cur ^ 1], (C*)A->gF);
+            k_adj_step<Tag, BC, BRINK><<<g, b, 0, A->stream>>>((const R*)slot(j), p, (const R*)A->lam[cur],
+                                                                (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
             D3Q19_HIPCHK(hipGetLastError());
             cur ^= 1;
         }
@@ -296,66 +398,120 @@ int run_backward(d3q19_adjoint* A) {
     return D3Q19_OK;
 }
 
+template <class Tag>
+int run_backward(d3q19_adjoint* A) {
+    const bool hw = A->cfg.boundary == D3Q19_BC_HALFWAY;
+    if (hw) return A->brink ? run_backward_t<Tag, BC_HALFWAY, true>(A) : run_backward_t<Tag, BC_HALFWAY, false>(A);
+    return A->brink ? run_backward_t<Tag, BC_PERIODIC, true>(A) : run_backward_t<Tag, BC_PERIODIC, false>(A);
+}
+
 void free_all(d3q19_adjoint* A) {
     for (void* p : A->ckpt)
         if (p) (void)hipFree(p);
     for (void* p : A->seg)
         if (p) (void)hipFree(p);
-    void* ps[] = {A->lam[0], A->lam[1], A->u0, A->force, A->gF, A->gu};
+    void* ps[] = {A->lam[0], A->lam[1], A->u0, A->force, A->s, A->gF, A->gu, A->gs};
     for (void* p : ps)
         if (p) (void)hipFree(p);
+}
+
+// Device bytes of the adjoint with checkpoint interval K.
+size_t plan_bytes(const d3q19_adjoint* A, int32_t K) {
+    const size_t nck = (size_t)((A->T + K - 1) / K);
+    const size_t nseg = (size_t)std::max(2, A->ckpt_same ? K - 1 : K);
+    const size_t vec = 3 * (size_t)A->ncell * A->es;
+    const size_t scal = (size_t)A->ncell * A->es;
+    return nck * A->ckpt_bytes + nseg * A->state_bytes + 2 * A->state_bytes + 4 * vec +
+           (A->brink ? 2 * scal : 0);
 }
 
 }  // namespace
 
 extern "C" {
 
-int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t K, void* stream,
+void d3q19_adjoint_options_default(d3q19_adjoint_options* opt) {
+    if (!opt) return;
+    std::memset(opt, 0, sizeof(*opt));
+    opt->steps = 1;
+    opt->checkpoint_every = 0;
+    opt->checkpoint_storage = -1;  // the lattice storage
+    opt->brinkman = 0;
+    opt->memory_budget = 0;
+}
+
+int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* opt, void* stream,
                          d3q19_adjoint** out) {
-    if (!cfg || !out) return set_error(D3Q19_EINVAL, "null argument");
+    if (!cfg || !opt || !out) return set_error(D3Q19_EINVAL, "null argument");
     *out = nullptr;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return set_error(D3Q19_EINVAL, "extents must be >= 1");
-    if (cfg->boundary != D3Q19_BC_PERIODIC) return set_error(D3Q19_EINVAL, "adjoint needs a periodic lattice");
+    if (cfg->boundary != D3Q19_BC_PERIODIC && cfg->boundary != D3Q19_BC_HALFWAY)
+        return set_error(D3Q19_EINVAL, "adjoint needs a periodic or half-way lattice");
+    if (cfg->boundary == D3Q19_BC_HALFWAY) {
+        if (cfg->wall_faces & ~63u) return set_error(D3Q19_EINVAL, "wall_faces has bits beyond the six faces");
+        for (int k = 0; k < 6; ++k)
+            for (int a = 0; a < 3; ++a)
+                if (cfg->face_velocity[k][a] != 0.0)
+                    return set_error(D3Q19_EINVAL, "adjoint supports stationary walls only");
+    }
     if (cfg->force_mode != D3Q19_FORCE_FIELD) return set_error(D3Q19_EINVAL, "adjoint needs force_mode FIELD");
     if (cfg->storage != D3Q19_F32 && cfg->storage != D3Q19_F64)
         return set_error(D3Q19_EINVAL, "adjoint storage must be F32 or F64");
     if (!(cfg->tau > 0.5)) return set_error(D3Q19_EINVAL, "tau must be > 0.5");
-    if (steps < 1) return set_error(D3Q19_EINVAL, "steps must be >= 1");
+    if (opt->steps < 1) return set_error(D3Q19_EINVAL, "steps must be >= 1");
+    const int32_t cks = opt->checkpoint_storage < 0 ? cfg->storage : opt->checkpoint_storage;
+    if (cks != cfg->storage && cks != D3Q19_F16)
+        return set_error(D3Q19_EINVAL, "checkpoint storage must be the lattice storage or F16");
     const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     if (ncell * 8 >= (1ull << 32)) return set_error(D3Q19_EINVAL, "lattice too large");
     if (cfg->nx > 65535 || cfg->ny > 65535) return set_error(D3Q19_EINVAL, "nx and ny must be <= 65535");
-    if (K <= 0) K = std::max(2, (int32_t)std::lround(std::sqrt((double)steps)));
-    K = std::min(K, steps);
     if (hipSetDevice(cfg->device) != hipSuccess) return set_error(D3Q19_EHIP, "hipSetDevice failed");
 
     d3q19_adjoint* A = new d3q19_adjoint();
     A->cfg = *cfg;
-    A->T = steps;
-    A->K = K;
-    A->nck = (steps + K - 1) / K;
+    A->opt = *opt;
+    A->T = opt->steps;
     A->ncell = (uint32_t)ncell;
     A->nzp = padded_nz((uint32_t)cfg->nz);
     A->es = cfg->storage == D3Q19_F64 ? 8 : 4;
-    A->row_bytes = (size_t)Q * A->nzp * A->es;
-    A->state_bytes = (size_t)cfg->nx * cfg->ny * A->row_bytes;
-    A->stream = (hipStream_t)stream;  // NULL: the default stream
-    const size_t state = A->state_bytes;
-    const size_t lam = A->state_bytes;  // same layout and element size
+    A->ckpt_same = cks == cfg->storage;
+    A->brink = opt->brinkman != 0;
+    const size_t rows = (size_t)cfg->nx * cfg->ny;
+    A->state_bytes = rows * Q * A->nzp * A->es;
+    A->ckpt_bytes = rows * Q * A->nzp * (A->ckpt_same ? A->es : 2);
+    int32_t K = opt->checkpoint_every;
+    if (K <= 0) {
+        K = 1;
+        for (int32_t k = 2; k <= A->T; ++k)
+            if (plan_bytes(A, k) < plan_bytes(A, K)) K = k;
+    }
+    K = std::min(K, A->T);
+    if (opt->memory_budget && plan_bytes(A, K) > opt->memory_budget) {
+        const size_t need = plan_bytes(A, K);
+        delete A;
+        return set_error(D3Q19_ENOMEM, "adjoint needs " + std::to_string(need) + " bytes with checkpoint interval " +
+                                           std::to_string(K) + ", over the memory budget");
+    }
+    A->K = K;
+    A->nck = (A->T + K - 1) / K;
+    A->stream = (hipStream_t)stream;
     const size_t vec = 3 * (size_t)ncell * A->es;
+    const size_t scal = (size_t)ncell * A->es;
     bool ok = true;
     A->ckpt.assign(A->nck, nullptr);
-    A->seg.assign(std::max(2, K - 1), nullptr);
-    for (auto& p : A->ckpt) ok = ok && hipMalloc(&p, state) == hipSuccess;
-    for (auto& p : A->seg) ok = ok && hipMalloc(&p, state) == hipSuccess;
-    ok = ok && hipMalloc(&A->lam[0], lam) == hipSuccess && hipMalloc(&A->lam[1], lam) == hipSuccess;
+    A->seg.assign(std::max(2, A->ckpt_same ? K - 1 : K), nullptr);
+    for (auto& p : A->ckpt) ok = ok && hipMalloc(&p, A->ckpt_bytes) == hipSuccess;
+    for (auto& p : A->seg) ok = ok && hipMalloc(&p, A->state_bytes) == hipSuccess;
+    ok = ok && hipMalloc(&A->lam[0], A->state_bytes) == hipSuccess &&
+         hipMalloc(&A->lam[1], A->state_bytes) == hipSuccess;
     ok = ok && hipMalloc(&A->u0, vec) == hipSuccess && hipMalloc(&A->force, vec) == hipSuccess;
     ok = ok && hipMalloc(&A->gF, vec) == hipSuccess && hipMalloc(&A->gu, vec) == hipSuccess;
+    if (A->brink) ok = ok && hipMalloc(&A->s, scal) == hipSuccess && hipMalloc(&A->gs, scal) == hipSuccess;
     if (!ok) {
         free_all(A);
         delete A;
         return set_error(D3Q19_ENOMEM, "adjoint checkpoint allocation failed");
     }
-    A->bytes = (A->ckpt.size() + A->seg.size()) * state + 2 * lam + 4 * vec;
+    A->bytes = plan_bytes(A, K);
     *out = A;
     return D3Q19_OK;
 }
@@ -374,18 +530,21 @@ int d3q19_adjoint_set_stream(d3q19_adjoint* A, void* stream) {
     return D3Q19_OK;
 }
 
-int d3q19_adjoint_forward(d3q19_adjoint* A, const void* u0, const void* force, void* u_final, int32_t dt,
-                          int32_t on_device) {
+int d3q19_adjoint_forward(d3q19_adjoint* A, const void* u0, const void* force, const void* brinkman,
+                          void* u_final, int32_t dt, int32_t on_device) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
     if (!u0 || !force || !u_final) return set_error(D3Q19_EINVAL, "null array");
+    if (A->brink && !brinkman) return set_error(D3Q19_EINVAL, "this adjoint needs a Brinkman field");
     if (dt != D3Q19_F64 && dt != D3Q19_F32) return set_error(D3Q19_EINVAL, "dtype must be F64 or F32");
     const size_t n3 = 3 * (size_t)A->ncell;
     if (int e = stage_in(u0, dt, on_device, n3, A->u0, ct_double(A), A->stream)) return e;
     if (int e = stage_in(force, dt, on_device, n3, A->force, ct_double(A), A->stream)) return e;
+    if (A->brink)
+        if (int e = stage_in(brinkman, dt, on_device, A->ncell, A->s, ct_double(A), A->stream)) return e;
     // u^T in the caller's dtype: written directly when on device, else via gu scratch
     void* dst = u_final;
     const bool direct = on_device != 0;
-    if (!direct) dst = A->gu;  // scratch in compute type
+    if (!direct) dst = A->gu;
     const int out_dt = direct ? dt : (ct_double(A) ? D3Q19_F64 : D3Q19_F32);
     int rc = ct_double(A) ? run_forward<TF64>(A, dst, out_dt) : run_forward<TF32>(A, dst, out_dt);
     if (rc) return rc;
@@ -399,10 +558,11 @@ int d3q19_adjoint_forward(d3q19_adjoint* A, const void* u0, const void* force, v
 }
 
 int d3q19_adjoint_backward(d3q19_adjoint* A, const void* grad_u_final, void* grad_u0, void* grad_force,
-                           int32_t dt, int32_t on_device) {
+                           void* grad_brinkman, int32_t dt, int32_t on_device) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
     if (!A->have_forward) return set_error(D3Q19_ESTATE, "adjoint_backward needs adjoint_forward first");
     if (!grad_u_final) return set_error(D3Q19_EINVAL, "null dL/du_final");
+    if (grad_brinkman && !A->brink) return set_error(D3Q19_EINVAL, "this adjoint has no Brinkman parameter");
     const size_t n3 = 3 * (size_t)A->ncell;
     if (int e = stage_in(grad_u_final, dt, on_device, n3, A->gu, ct_double(A), A->stream)) return e;
     int rc = ct_double(A) ? run_backward<TF64>(A) : run_backward<TF32>(A);
@@ -411,6 +571,8 @@ int d3q19_adjoint_backward(d3q19_adjoint* A, const void* grad_u_final, void* gra
         if (int e = stage_out(A->gu, ct_double(A), n3, grad_u0, dt, on_device, A->stream)) return e;
     if (grad_force)
         if (int e = stage_out(A->gF, ct_double(A), n3, grad_force, dt, on_device, A->stream)) return e;
+    if (grad_brinkman)
+        if (int e = stage_out(A->gs, ct_double(A), A->ncell, grad_brinkman, dt, on_device, A->stream)) return e;
     D3Q19_HIPCHK(hipStreamSynchronize(A->stream));
     return D3Q19_OK;
 }

@@ -124,7 +124,7 @@ StepGrid step_grid(const d3q19_lattice* L) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
     const uint32_t block = step_block<Tag>();
-    const uint32_t bz = nz >= 256 ? 256 : (nz >= 128 ? 128 : 64);
+    const uint32_t bz = row_block(nz);
     const uint32_t by = block / bz;
     sg.block = dim3(bz, by, 1);
     sg.grid = dim3((nz + bz - 1) / bz, (ny + by - 1) / by, nx);

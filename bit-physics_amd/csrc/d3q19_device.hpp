@@ -144,6 +144,20 @@ inline void set_collision_constants(Params<C>& p, double tau) {
     }
 }
 
+// Threads of a block along a row: the widest of 256, 128, 64 that leaves the
+// fewest idle lanes on a row of nz cells (384 -> 128 x 3, 320 -> 64 x 5).
+__host__ __device__ inline uint32_t row_block(uint32_t nz) {
+    uint32_t best = 64, waste = ~0u;
+    for (uint32_t b = 256; b >= 64; b /= 2) {
+        const uint32_t w = (nz + b - 1) / b * b - nz;
+        if (w < waste) {
+            waste = w;
+            best = b;
+        }
+    }
+    return best;
+}
+
 // Padded row length for nz.
 __host__ __device__ inline uint32_t padded_nz(uint32_t nz) { return (nz + 63u) / 64u * 64u; }
 

@@ -31,7 +31,8 @@ EXPORTED = (
     "d3q19_diagnostics", "d3q19_enstrophy", "d3q19_synchronize", "d3q19_step_count",
     "d3q19_device_bytes", "d3q19_bgk_step", "d3q19_stream", "d3q19_bounce_back_y_walls",
     "d3q19_feq_field", "d3q19_density_field", "d3q19_momentum_field",
-    "d3q19_macroscopic_velocity", "d3q19_adjoint_create", "d3q19_adjoint_destroy",
+    "d3q19_macroscopic_velocity", "d3q19_adjoint_options_default", "d3q19_adjoint_create",
+    "d3q19_adjoint_destroy",
     "d3q19_adjoint_set_stream", "d3q19_adjoint_forward", "d3q19_adjoint_backward",
     "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
 )
@@ -62,6 +63,18 @@ class Config(ctypes.Structure):
         ("force_mode", ctypes.c_int32),
         ("force", ctypes.c_double * 3),
         ("device", ctypes.c_int32),
+    ]
+
+
+class AdjointOptions(ctypes.Structure):
+    """Mirror of d3q19_adjoint_options (include/d3q19.h)."""
+
+    _fields_ = [
+        ("steps", ctypes.c_int32),
+        ("checkpoint_every", ctypes.c_int32),
+        ("checkpoint_storage", ctypes.c_int32),
+        ("brinkman", ctypes.c_int32),
+        ("memory_budget", ctypes.c_uint64),
     ]
 
 
@@ -109,11 +122,12 @@ def load() -> ctypes.CDLL:
         "d3q19_density_field": (ctypes.c_int, [P, P, i64, i32, P]),
         "d3q19_momentum_field": (ctypes.c_int, [P, P, i64, i32, P]),
         "d3q19_macroscopic_velocity": (ctypes.c_int, [P, P, P, i64, i32, P]),
-        "d3q19_adjoint_create": (ctypes.c_int, [ctypes.POINTER(Config), i32, i32, P, PP]),
+        "d3q19_adjoint_options_default": (None, [ctypes.POINTER(AdjointOptions)]),
+        "d3q19_adjoint_create": (ctypes.c_int, [ctypes.POINTER(Config), ctypes.POINTER(AdjointOptions), P, PP]),
         "d3q19_adjoint_destroy": (ctypes.c_int, [P]),
         "d3q19_adjoint_set_stream": (ctypes.c_int, [P, P]),
-        "d3q19_adjoint_forward": (ctypes.c_int, [P, P, P, P, i32, i32]),
-        "d3q19_adjoint_backward": (ctypes.c_int, [P, P, P, P, i32, i32]),
+        "d3q19_adjoint_forward": (ctypes.c_int, [P, P, P, P, P, i32, i32]),
+        "d3q19_adjoint_backward": (ctypes.c_int, [P, P, P, P, P, i32, i32]),
         "d3q19_adjoint_checkpoint_every": (i32, [P]),
         "d3q19_adjoint_memory_bytes": (ctypes.c_size_t, [P]),
         "d3q19_copy_kernel": (ctypes.c_int, [P, P, ctypes.c_size_t, P]),

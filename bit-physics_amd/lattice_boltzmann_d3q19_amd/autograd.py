@@ -5,12 +5,14 @@
     loss = ((u_T - u_target) ** 2).sum(0).mean()
     loss.backward()                     # u0.grad, F.grad via the adjoint kernels
 
-Forward: f^0 = feq(1, u0); ``steps`` periodic BGK+Guo steps with the per-cell
-force F (the reference's ``bgk_step(f, tau, force_lattice=F)``,
-packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/bgk.py:45-112);
-u^T = ``macroscopic_velocity(f^T, F)`` (bgk.py:129-142).  Backward: the
-hand-written adjoint kernel with checkpoint-every-K recomputation.
-Tensors are passed by pointer on the current HIP stream.
+Forward: f^0 = feq(1, u0); ``steps`` BGK+Guo steps with the per-cell force F
+(the reference's ``bgk_step(f, tau, force_lattice=F)``,
+packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/bgk.py:45-112),
+optionally with half-way bounce-back walls and a Brinkman solid fraction s
+(body force F - s m); u^T = ``macroscopic_velocity(f^T, F - s m)``
+(bgk.py:129-142).  Backward: the hand-written adjoint kernel with
+checkpoint-every-K recomputation.  Tensors are passed by pointer on the
+current HIP stream.
 """
 
 from __future__ import annotations
@@ -18,21 +20,47 @@ from __future__ import annotations
 import ctypes
 
 from . import _device as dev
-from ._lib import D3Q19_F32, D3Q19_F64, FORCE_FIELD, STORAGE, Config, call, load
+from ._lib import (
+    BC_HALFWAY,
+    BC_PERIODIC,
+    D3Q19_F16,
+    D3Q19_F32,
+    D3Q19_F64,
+    FACES,
+    FORCE_FIELD,
+    STORAGE,
+    AdjointOptions,
+    Config,
+    call,
+    load,
+)
 
 
 class DifferentiableLBM:
-    """A differentiable D3Q19 forward map u0, F -> u^T on one GPU.
+    """A differentiable D3Q19 forward map (u0, F[, s]) -> u^T on one GPU.
+
+    Parameters
+    ----------
+    shape, tau, steps : lattice, relaxation time, number of steps T.
+    checkpoint_every : K; <= 0 picks the K that needs the least memory.
+    storage : "f32" or "f64" -- working precision of forward and adjoint.
+    boundary : "periodic" or "halfway" (stationary walls on ``wall_faces``).
+    brinkman : if True the forward map takes a Brinkman solid fraction s.
+    checkpoint_storage : None (= storage) or "f16" (f - w checkpoints).
+    memory_budget : bytes the adjoint may hold (0 = unlimited); creation
+        fails if no checkpoint interval fits.
 
     Holds the checkpoints of the most recent forward call: one backward per
     forward, in order.
     """
 
-    def __init__(self, shape, tau, steps, checkpoint_every=0, storage="f32", device=0):
+    def __init__(self, shape, tau, steps, checkpoint_every=0, storage="f32", boundary="periodic",
+                 wall_faces=(), brinkman=False, checkpoint_storage=None, memory_budget=0, device=0):
         lib = load()
         self.shape = tuple(int(s) for s in shape)
         self.steps = int(steps)
         self.storage = storage
+        self.brinkman = bool(brinkman)
         self.device = int(device)
         cfg = Config()
         lib.d3q19_config_default(ctypes.byref(cfg))
@@ -40,10 +68,22 @@ class DifferentiableLBM:
         cfg.tau = float(tau)
         cfg.storage = STORAGE[storage]
         cfg.force_mode = FORCE_FIELD
+        cfg.boundary = {"periodic": BC_PERIODIC, "halfway": BC_HALFWAY}[boundary]
+        mask = 0
+        for face in wall_faces:
+            mask |= 1 << FACES[face]
+        cfg.wall_faces = mask
         cfg.device = self.device
+        opt = AdjointOptions()
+        lib.d3q19_adjoint_options_default(ctypes.byref(opt))
+        opt.steps = self.steps
+        opt.checkpoint_every = int(checkpoint_every)
+        opt.checkpoint_storage = -1 if checkpoint_storage is None else {"f16": D3Q19_F16, **STORAGE}[checkpoint_storage]
+        opt.brinkman = 1 if self.brinkman else 0
+        opt.memory_budget = int(memory_budget)
         h = ctypes.c_void_p()
-        call("d3q19_adjoint_create", ctypes.byref(cfg), self.steps, int(checkpoint_every),
-             dev.stream_ptr(self.device), ctypes.byref(h))
+        call("d3q19_adjoint_create", ctypes.byref(cfg), ctypes.byref(opt), dev.stream_ptr(self.device),
+             ctypes.byref(h))
         self._h = h
         self.tdtype = dev.torch().float64 if storage == "f64" else dev.torch().float32
 
@@ -66,37 +106,51 @@ class DifferentiableLBM:
         except Exception:
             pass
 
-    def _check(self, t, name):
-        if tuple(t.shape) != (3,) + self.shape:
-            raise ValueError(f"{name} must have shape {(3,) + self.shape}, got {tuple(t.shape)}")
+    def _check(self, t, shape, name):
+        if tuple(t.shape) != shape:
+            raise ValueError(f"{name} must have shape {shape}, got {tuple(t.shape)}")
         if not t.is_cuda:
             raise ValueError(f"{name} must be a device tensor")
 
-    def forward_raw(self, u0, force):
-        """u^T for (u0, F) without autograd bookkeeping (keeps checkpoints)."""
+    def _code(self):
+        return D3Q19_F64 if self.tdtype == dev.torch().float64 else D3Q19_F32
+
+    def forward_raw(self, u0, force, s=None):
+        """u^T for (u0, F[, s]) without autograd bookkeeping (keeps checkpoints)."""
         t = dev.torch()
-        self._check(u0, "u0")
-        self._check(force, "force")
+        v3 = (3,) + self.shape
+        self._check(u0, v3, "u0")
+        self._check(force, v3, "force")
         u0c = u0.detach().to(self.tdtype).contiguous()
         Fc = force.detach().to(self.tdtype).contiguous()
+        sc = None
+        if self.brinkman:
+            if s is None:
+                raise ValueError("this solver takes a Brinkman field s")
+            self._check(s, self.shape, "s")
+            sc = s.detach().to(self.tdtype).contiguous()
         out = t.empty_like(u0c)
-        code = D3Q19_F64 if self.tdtype == t.float64 else D3Q19_F32
         call("d3q19_adjoint_set_stream", self._h, dev.stream_ptr(self.device))
-        call("d3q19_adjoint_forward", self._h, dev.ptr(u0c), dev.ptr(Fc), dev.ptr(out), code, 1)
+        call("d3q19_adjoint_forward", self._h, dev.ptr(u0c), dev.ptr(Fc), dev.ptr(sc), dev.ptr(out),
+             self._code(), 1)
         return out
 
     def backward_raw(self, grad_u):
+        """(dL/du0, dL/dF, dL/ds or None) for dL/du^T of the last forward."""
         t = dev.torch()
         g = grad_u.detach().to(self.tdtype).contiguous()
         gu0 = t.empty_like(g)
         gF = t.empty_like(g)
-        code = D3Q19_F64 if self.tdtype == t.float64 else D3Q19_F32
+        gs = t.empty(self.shape, dtype=self.tdtype, device=g.device) if self.brinkman else None
         call("d3q19_adjoint_set_stream", self._h, dev.stream_ptr(self.device))
-        call("d3q19_adjoint_backward", self._h, dev.ptr(g), dev.ptr(gu0), dev.ptr(gF), code, 1)
-        return gu0, gF
+        call("d3q19_adjoint_backward", self._h, dev.ptr(g), dev.ptr(gu0), dev.ptr(gF), dev.ptr(gs),
+             self._code(), 1)
+        return gu0, gF, gs
 
-    def __call__(self, u0, force):
-        return _LBMFunction.apply(u0, force, self)
+    def __call__(self, u0, force, s=None):
+        if self.brinkman:
+            return _LBMFunction.apply(u0, force, s, self)
+        return _LBMFunction.apply(u0, force, None, self)
 
 
 def _function_class():
@@ -104,15 +158,16 @@ def _function_class():
 
     class LBMFunction(torch.autograd.Function):
         @staticmethod
-        def forward(ctx, u0, force, solver):
+        def forward(ctx, u0, force, s, solver):
             ctx.solver = solver
-            ctx.in_dtypes = (u0.dtype, force.dtype)
-            return solver.forward_raw(u0, force)
+            ctx.in_dtypes = (u0.dtype, force.dtype, None if s is None else s.dtype)
+            return solver.forward_raw(u0, force, s)
 
         @staticmethod
         def backward(ctx, grad_u):
-            gu0, gF = ctx.solver.backward_raw(grad_u)
-            return gu0.to(ctx.in_dtypes[0]), gF.to(ctx.in_dtypes[1]), None
+            gu0, gF, gs = ctx.solver.backward_raw(grad_u)
+            d0, d1, d2 = ctx.in_dtypes
+            return gu0.to(d0), gF.to(d1), (None if gs is None else gs.to(d2)), None
 
     return LBMFunction
 
@@ -134,3 +189,9 @@ def velocity_matching_loss(u_final, u_target):
     t = dev.torch()
     d = (u_final.to(t.float64) - u_target.to(t.float64))
     return (d * d).sum(0).mean()
+
+
+def outlet_velocity(u_final):
+    """Mean x-velocity over the last x-plane (the outlet of a channel along x), FP64."""
+    t = dev.torch()
+    return u_final[0, -1].to(t.float64).mean()

@@ -78,3 +78,33 @@ def spectral_divergence(u) -> float:
     div = kx * uh[0] + ky * uh[1] + kz * uh[2]
     mag = np.sqrt(kx * kx + ky * ky + kz * kz) * np.sqrt(sum(np.abs(c) ** 2 for c in uh))
     return float(np.max(np.abs(div)) / max(np.max(mag), 1e-300))
+
+
+def random_spheres(shape, r_min=6.0, r_max=12.0, volume_fraction=0.05, seed=42, wall_axis=1):
+    """Seeded random spherical obstacles as a Brinkman solid fraction in [0, 1].
+
+    Spheres of radius r in [r_min, r_max] are placed at uniform random centres
+    (periodic along the axes other than ``wall_axis``; kept clear of the two
+    walls along ``wall_axis``) until the solid volume reaches
+    ``volume_fraction``.  The fraction falls linearly from 1 to 0 over one
+    cell at the sphere surface: s = clip(r + 0.5 - distance, 0, 1).
+    """
+    rng = np.random.default_rng(seed)
+    n = np.asarray(shape)
+    s = np.zeros(shape, np.float64)
+    grids = np.meshgrid(*[np.arange(k, dtype=np.float64) for k in shape], indexing="ij")
+    target = volume_fraction * s.size
+    for _ in range(100000):
+        if s.sum() >= target:
+            break
+        r = rng.uniform(r_min, r_max)
+        c = rng.uniform(0, n)
+        c[wall_axis] = rng.uniform(r + 1, n[wall_axis] - r - 2)
+        d2 = np.zeros(shape)
+        for a in range(3):
+            d = grids[a] - c[a]
+            if a != wall_axis:
+                d = d - n[a] * np.round(d / n[a])  # periodic distance
+            d2 += d * d
+        s = np.maximum(s, np.clip(r + 0.5 - np.sqrt(d2), 0.0, 1.0))
+    return s

@@ -142,27 +142,39 @@ int d3q19_macroscopic_velocity(const void* f, const void* force, void* u, int64_
                                int32_t dtype, void* stream);
 
 /* ---- differentiable step (no reference counterpart) ------------------
- * Forward map on a periodic lattice with a per-cell Guo force F:
- *   f^0 = feq_field(1, u0);  f^{n+1} = bgk_step(f^n, tau, F)   (n < steps)
- *   u^T = macroscopic_velocity(f^T, F)
- * The backward pass propagates dL/du^T to dL/du0 and dL/dF with a
- * hand-written adjoint kernel (reverse streaming + transposed BGK/Guo
- * Jacobian), recomputing the states between checkpoints taken every
- * `checkpoint_every` steps (<= 0: chosen to minimise memory).
- * cfg must be periodic with force_mode D3Q19_FORCE_FIELD; storage F32 or F64
- * (the adjoint runs in the storage precision).  Fields are device or host
- * arrays of dtype F32/F64 in (3, Nx, Ny, Nz) layout. */
+ * Forward map with a per-cell Guo force F and an optional Brinkman solid
+ * fraction s (body force F - s m):
+ *   f^0 = feq_field(1, u0);  f^{n+1} = bgk_step(f^n, tau, F - s m) [+ walls]
+ *   u^T = macroscopic_velocity(f^T, F - s m)
+ * The backward pass propagates dL/du^T to dL/du0, dL/dF and dL/ds with a
+ * hand-written adjoint kernel (reverse streaming with the bounce-back links
+ * reversed + transposed BGK/Guo/Brinkman Jacobian), recomputing the states
+ * between checkpoints.  cfg: force_mode D3Q19_FORCE_FIELD, boundary PERIODIC
+ * or HALFWAY with stationary walls and no solid cells, storage F32 or F64
+ * (the adjoint runs in the storage precision).  Fields are (3,Nx,Ny,Nz) or
+ * (Nx,Ny,Nz) device or host arrays of dtype F32/F64. */
 typedef struct d3q19_adjoint d3q19_adjoint;
-int d3q19_adjoint_create(const d3q19_config* cfg, int32_t steps, int32_t checkpoint_every,
-                         void* stream, d3q19_adjoint** out);
+
+typedef struct d3q19_adjoint_options {
+    int32_t steps;               /* T >= 1 */
+    int32_t checkpoint_every;    /* K; <= 0: the K that needs the least memory */
+    int32_t checkpoint_storage;  /* the lattice storage (default), or D3Q19_F16 (f - w) */
+    int32_t brinkman;            /* nonzero: s is a parameter of the forward map */
+    uint64_t memory_budget;      /* bytes the adjoint may hold, 0 = unlimited */
+} d3q19_adjoint_options;
+
+void d3q19_adjoint_options_default(d3q19_adjoint_options* opt);
+int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* opt, void* stream,
+                         d3q19_adjoint** out);
 int d3q19_adjoint_destroy(d3q19_adjoint* adj);
 int d3q19_adjoint_set_stream(d3q19_adjoint* adj, void* stream);
-/* Run the forward map from (u0, F), keep checkpoints, write u^T. */
-int d3q19_adjoint_forward(d3q19_adjoint* adj, const void* u0, const void* force, void* u_final,
-                          int32_t dtype, int32_t on_device);
-/* Given dL/du^T, write dL/du0 and dL/dF (either may be NULL). Needs a forward first. */
+/* Run the forward map from (u0, F, s), keep checkpoints, write u^T.  s is
+   ignored (may be NULL) unless the options enable the Brinkman parameter. */
+int d3q19_adjoint_forward(d3q19_adjoint* adj, const void* u0, const void* force, const void* brinkman,
+                          void* u_final, int32_t dtype, int32_t on_device);
+/* Given dL/du^T, write dL/du0, dL/dF and dL/ds (any may be NULL). Needs a forward first. */
 int d3q19_adjoint_backward(d3q19_adjoint* adj, const void* grad_u_final, void* grad_u0,
-                           void* grad_force, int32_t dtype, int32_t on_device);
+                           void* grad_force, void* grad_brinkman, int32_t dtype, int32_t on_device);
 int32_t d3q19_adjoint_checkpoint_every(const d3q19_adjoint* adj);
 size_t d3q19_adjoint_memory_bytes(const d3q19_adjoint* adj);
 

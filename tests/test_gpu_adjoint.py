@@ -118,3 +118,83 @@ def test_adjoint_errors(mods):
         s.backward_raw(torch.zeros((3, 4, 4, 4), device="cuda"))
     with pytest.raises(D3Q19Error, match="EINVAL"):
         ag.DifferentiableLBM((4, 4, 4), 0.6, 5, storage="f16")
+
+
+def channel_problem(shape, seed=5):
+    rng = np.random.default_rng(seed)
+    u0 = 0.01 * rng.standard_normal((3,) + shape)
+    u0[:, :, 0, :] = 0.0
+    u0[:, :, -1, :] = 0.0
+    F = np.zeros((3,) + shape)
+    F[0] = 1e-4
+    s = np.clip(rng.random(shape) * 1.4 - 0.6, 0.0, 1.0)
+    return u0, F, s
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-6), ("f32", 2e-4)])
+def test_halfway_brinkman_adjoint_matches_central_differences(mods, storage, tol):
+    """Walls on -y/+y, Brinkman field, per-cell force: <grad, d> vs FP64 central differences."""
+    torch, ag = mods
+    print_case("adjoint-halfway-brinkman-fd", storage=storage)
+    shape = (10, 9, 8)
+    T, K, tau = 30, 7, 0.7
+    u0, F, s = channel_problem(shape)
+    kw = dict(boundary="halfway", wall_faces=("-y", "+y"), brinkman=True, checkpoint_every=K)
+    solver = ag.DifferentiableLBM(shape, tau, T, storage=storage, **kw)
+    a = torch.tensor(u0, device="cuda", requires_grad=True)
+    b = torch.tensor(F, device="cuda", requires_grad=True)
+    c = torch.tensor(s, device="cuda", requires_grad=True)
+    J = ag.outlet_velocity(solver(a, b, c))
+    J.backward()
+    gu, gF, gs = (t.grad.double().cpu().numpy() for t in (a, b, c))
+    fwd = ag.DifferentiableLBM(shape, tau, T, storage="f64", **kw)
+
+    def Jat(x, y, z):
+        return float(ag.outlet_velocity(fwd.forward_raw(*(torch.tensor(v, device="cuda") for v in (x, y, z)))))
+
+    rng = np.random.default_rng(11)
+    eps = 1e-5 if storage == "f64" else 1e-3
+    fds, ads = [], []
+    for _ in range(4):
+        du = rng.standard_normal(u0.shape) * 0.1
+        dF = rng.standard_normal(F.shape) * 1e-3
+        ds = rng.standard_normal(s.shape)
+        fds.append((Jat(u0 + eps * du, F + eps * dF, s + eps * ds) - Jat(u0 - eps * du, F - eps * dF, s - eps * ds)) / (2 * eps))
+        ads.append(float(np.sum(gu * du) + np.sum(gF * dF) + np.sum(gs * ds)))
+    scale = max(abs(v) for v in fds)
+    for k, (ad, fd) in enumerate(zip(ads, fds)):
+        assert abs(ad - fd) <= tol * scale, (k, ad, fd)
+
+
+def test_fp16_checkpoints_close_to_fp32_checkpoints(mods):
+    torch, ag = mods
+    print_case("adjoint-fp16-checkpoints")
+    shape = (16, 12, 10)
+    u0, F, s = channel_problem(shape, seed=9)
+    kw = dict(boundary="halfway", wall_faces=("-y", "+y"), brinkman=True, checkpoint_every=8)
+    outs = {}
+    for cks in (None, "f16"):
+        solver = ag.DifferentiableLBM(shape, 0.7, 40, checkpoint_storage=cks, **kw)
+        uT = solver.forward_raw(*(torch.tensor(v, device="cuda") for v in (u0, F, s)))
+        g = torch.zeros_like(uT)
+        g[0, -1] = 1.0 / (shape[1] * shape[2])
+        outs[cks] = (uT.cpu().numpy(), [t.cpu().numpy() for t in solver.backward_raw(g)])
+    # the forward never reads checkpoints; the gradients see the rounded states
+    assert np.array_equal(outs[None][0], outs["f16"][0])
+    for a, b in zip(outs[None][1], outs["f16"][1]):
+        assert field_scaled_err(b, a) < 2e-3
+
+
+def test_memory_budget_autotune(mods):
+    torch, ag = mods
+    from lattice_boltzmann_d3q19_amd import D3Q19Error
+
+    print_case("adjoint-budget")
+    shape = (16, 16, 16)
+    s = ag.DifferentiableLBM(shape, 0.6, 400, checkpoint_every=0, checkpoint_storage="f16",
+                             memory_budget=1 << 30)
+    K = s.checkpoint_every
+    assert 10 <= K <= 30 and s.memory_bytes <= 1 << 30
+    s.close()
+    with pytest.raises(D3Q19Error, match="ENOMEM"):
+        ag.DifferentiableLBM(shape, 0.6, 400, checkpoint_every=0, memory_budget=1 << 20)
