@@ -91,6 +91,12 @@ def load() -> ctypes.CDLL:
             f"{LIB_PATH} is missing: build it with `python bit-physics_amd/build.py` "
             "(the HIP library is the only compute path)"
         )
+    # torch ships its own HIP runtime; load it first so that libd3q19 binds to
+    # the same libamdhip64 instead of bringing a second runtime into the process
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(str(LIB_PATH))
     P, i32, i64, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
     PP = ctypes.POINTER(ctypes.c_void_p)

@@ -97,6 +97,42 @@ def test_ywalls_long_run_matches_oracle(amd, storage):
     assert field_scaled_err(lat.populations(), ref) < TOL[storage]
 
 
+# 16-bit storage keeps f - w in FP16 (scaled by 2048 / w_i) or BF16 and
+# collides in FP32.  Measured against the FP64 oracle after 200 steps:
+# deviation error (max |d - d_ref| / max |d_ref|) FP16 1.0e-3, BF16 8.1e-3;
+# KE relative error FP16 3.7e-5, BF16 1.1e-3.  The bounds carry ~3x headroom.
+SIXTEEN = {"f16": {"dev": 3e-3, "ke": 2e-4, "mass": 1e-6},
+           "bf16": {"dev": 2.5e-2, "ke": 4e-3, "mass": 1e-5}}
+
+
+@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("case", ["tg-periodic", "ywalls-moving"])
+def test_sixteen_bit_storage_tracks_fp64_oracle(amd, storage, case):
+    from lattice_boltzmann_d3q19_amd.initial import taylor_green
+
+    print_case("lattice-16bit", storage=storage, case=case)
+    _, w, _, _ = oracle.constants()
+    if case == "tg-periodic":
+        shape, kw, okw = (32, 32, 32), {}, {}
+    else:
+        shape = (24, 20, 12)
+        kw = {"boundary": "ywalls", "wall_velocity_top": (0.03, 0.0, 0.0), "force": (1e-5, 0.0, 0.0)}
+        F = np.broadcast_to(np.array([1e-5, 0.0, 0.0])[:, None, None, None], (3,) + shape)
+        okw = {"force_lattice": F, "ywalls": True, "wall_velocity_top": (0.03, 0.0, 0.0)}
+    rho, u = taylor_green(shape, 0.04)
+    f0 = oracle.feq_field(rho, u)
+    ref = oracle.run_reference(f0, 0.6, 200, **okw)
+    lat = amd.Lattice(shape, 0.6, storage=storage, **kw)
+    lat.set_populations(f0).step(200)
+    wb = w[:, None, None, None]
+    tol = SIXTEEN[storage]
+    assert field_scaled_err(lat.populations() - wb, ref - wb) < tol["dev"]
+    dref = oracle.diagnostics(ref, force_uniform=kw.get("force"))
+    diag = lat.diagnostics()
+    assert abs(diag["kinetic_energy"] - dref[4]) / dref[4] < tol["ke"]
+    assert abs(diag["mass"] - dref[0]) / dref[0] < tol["mass"]
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32"])
 def test_poiseuille_known_answer_on_gpu(amd, storage):
     """equivalence.md:240-241: max|u| 8.6508320e-03, max rho 1.0000000 at step 1000."""
@@ -252,3 +288,19 @@ def test_state_errors(amd):
         lat.set_solid(np.zeros((4, 4, 4), np.uint8))  # periodic lattice has no solids
     with pytest.raises(ValueError):
         lat.set_populations(np.zeros((19, 4, 4, 5)))
+
+
+def test_library_loaded_before_torch_shares_one_hip_runtime():
+    """Import order must not matter: a fresh process that loads the package first."""
+    import subprocess
+    import sys
+
+    from conftest import PKG_DIR, ROOT
+
+    print_case("lattice-import-order")
+    code = ("import sys; sys.path[:0] = [%r, %r]; import numpy as np; "
+            "import lattice_boltzmann_d3q19_amd as amd; "
+            "lat = amd.Lattice((8, 8, 8), 0.6); lat.set_equilibrium(); lat.step(3); "
+            "r, u = lat.moments(); assert abs(r.mean() - 1) < 1e-12; print('ok')") % (str(ROOT), str(PKG_DIR))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=110)
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
