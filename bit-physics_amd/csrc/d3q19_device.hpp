@@ -28,6 +28,24 @@ __host__ __device__ constexpr T wgt(int i) {
     return i == 0 ? T(1) / T(3) : (i <= 6 ? T(1) / T(18) : T(1) / T(36));
 }
 
+// c_i . (x, y, z) with the zero components left out (every component of c_i
+// is 0 or +-1), so no product with zero is formed.
+template <class C>
+__device__ __forceinline__ C cdot(int i, C x, C y, C z) {
+    C r = C(0);
+    bool any = false;
+    if (CX[i] != 0) {
+        r = CX[i] > 0 ? x : -x;
+        any = true;
+    }
+    if (CY[i] != 0) {
+        r = any ? (CY[i] > 0 ? r + y : r - y) : (CY[i] > 0 ? y : -y);
+        any = true;
+    }
+    if (CZ[i] != 0) r = any ? (CZ[i] > 0 ? r + z : r - z) : (CZ[i] > 0 ? z : -z);
+    return r;
+}
+
 enum { BC_PERIODIC = 0, BC_YWALLS = 1, BC_HALFWAY = 2 };
 enum { FORCE_NONE = 0, FORCE_UNIFORM = 1, FORCE_FIELD = 2 };
 
@@ -219,6 +237,44 @@ __device__ __forceinline__ void store_local(void* out, const Params<typename Cod
     for (int i = 0; i < Q; ++i) st<Tag>(row, i, p.dir_bytes, z * ES, d[i]);
 }
 
+// Reference y-wall treatment (reference/bgk.py:190-215) applied in registers
+// to the periodically streamed deviations of a cell in row y: at y = 0 the
+// populations with c_y > 0 (at y = ny - 1 those with c_y < 0) are replaced by
+// their opposites minus 6 w_i rho_wall (c_i . u_wall), rho_wall being the
+// density of all 19 streamed values before the swap.
+template <class C>
+__device__ __forceinline__ void ywall_swap(const Params<C>& p, uint32_t y, C d[Q]) {
+#pragma clang fp contract(off)
+    const bool bot = (y == 0), top = (y + 1 == p.ny);
+    if (bot || top) {
+        C rw = C(0);
+#pragma unroll
+        for (int i = 0; i < Q; ++i) rw += d[i];
+        rw += C(1);
+        C nd[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) nd[i] = d[i];
+        if (bot) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                if (CY[i] > 0) {
+                    const C cu = cdot<C>(i, p.wbot[0], p.wbot[1], p.wbot[2]);
+                    nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
+                }
+        }
+        if (top) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i)
+                if (CY[i] < 0) {
+                    const C cu = cdot<C>(i, p.wtop[0], p.wtop[1], p.wtop[2]);
+                    nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
+                }
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d[i] = nd[i];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Pull streaming with boundary handling.  Produces the pre-collision
 // deviations d_i(x) = f_i(x) - w_i of the reference state after streaming and
@@ -228,6 +284,7 @@ __device__ __forceinline__ void store_local(void* out, const Params<typename Cod
 template <class Tag, int BC>
 __device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                      uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
+#pragma clang fp contract(off)
     typedef typename Codec<Tag>::C C;
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};
@@ -242,37 +299,7 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
             d[i] = ld<Tag>(row, i, DB, zs[1 - CZ[i]] * ES);
         }
     }
-    if constexpr (BC == BC_YWALLS) {
-        // reference/bgk.py:190-215 on the post-stream values, in registers.
-        const bool bot = (y == 0), top = (y + 1 == p.ny);
-        if (bot || top) {
-            C rw = C(0);
-#pragma unroll
-            for (int i = 0; i < Q; ++i) rw += d[i];
-            rw += C(1);
-            C nd[Q];
-#pragma unroll
-            for (int i = 0; i < Q; ++i) nd[i] = d[i];
-            if (bot) {
-#pragma unroll
-                for (int i = 0; i < Q; ++i)
-                    if (CY[i] > 0) {
-                        C cu = C(CX[i]) * p.wbot[0] + C(CY[i]) * p.wbot[1] + C(CZ[i]) * p.wbot[2];
-                        nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
-                    }
-            }
-            if (top) {
-#pragma unroll
-                for (int i = 0; i < Q; ++i)
-                    if (CY[i] < 0) {
-                        C cu = C(CX[i]) * p.wtop[0] + C(CY[i]) * p.wtop[1] + C(CZ[i]) * p.wtop[2];
-                        nd[i] = d[OPP[i]] - C(6) * wgt<C>(i) * rw * cu;
-                    }
-            }
-#pragma unroll
-            for (int i = 0; i < Q; ++i) d[i] = nd[i];
-        }
-    }
+    if constexpr (BC == BC_YWALLS) ywall_swap<C>(p, y, d);
     if constexpr (BC == BC_HALFWAY) {
         const uint32_t wm = p.wall_mask;
         const char* self = row_of(in, p, x, y);
@@ -286,8 +313,7 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
             if (CZ[i] > 0 && z == 0 && (wm & 16u)) face = 4;
             if (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u)) face = 5;
             if (face >= 0) {
-                C cu = C(CX[i]) * p.face_u[face][0] + C(CY[i]) * p.face_u[face][1] +
-                       C(CZ[i]) * p.face_u[face][2];
+                const C cu = cdot<C>(i, p.face_u[face][0], p.face_u[face][1], p.face_u[face][2]);
                 d[i] = ld<Tag>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;
             } else {
                 const uint32_t sx = xs[1 - CX[i]], sy = ys[1 - CY[i]], sz = zs[1 - CZ[i]];
@@ -303,6 +329,7 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
 // Moments of the deviations: drho = rho - 1, m = sum c_i f_i.
 template <class C>
 __device__ __forceinline__ void moments(const C d[Q], C& drho, C& mx, C& my, C& mz) {
+#pragma clang fp contract(off)
     C r = C(0), a = C(0), b = C(0), c = C(0);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -324,6 +351,7 @@ __device__ __forceinline__ void moments(const C d[Q], C& drho, C& mx, C& my, C& 
 template <class C>
 __device__ __forceinline__ void cell_force(const Params<C>& p, uint32_t cell, C mx, C my, C mz,
                                            C& fx, C& fy, C& fz) {
+#pragma clang fp contract(off)
     fx = C(0);
     fy = C(0);
     fz = C(0);
@@ -356,9 +384,14 @@ __device__ __forceinline__ void cell_force(const Params<C>& p, uint32_t cell, C 
 // the three axis pairs take the momentum residual.  FORCED = false compiles
 // the force terms out.  Returns rho and the Guo-corrected velocity
 // (macroscopic_velocity, bgk.py:129-142).
+//
+// Contraction is off and every fused multiply-add is written out, so the
+// result of a cell is fixed by this source alone: the scalar and vector-row
+// steps, the read-back and the adjoint's recomputation agree bit for bit.
 template <class C, bool FORCED = true>
 __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cell, C& rho_o,
                                         C& ux_o, C& uy_o, C& uz_o) {
+#pragma clang fp contract(off)
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
@@ -366,20 +399,27 @@ __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cel
     if constexpr (FORCED) cell_force(p, cell, mx, my, mz, fx, fy, fz);
     const C rs = rho > C(1e-30) ? rho : C(1e-30);
     const C irs = C(1) / rs;
-    const C ux = (mx + C(0.5) * fx) * irs;
-    const C uy = (my + C(0.5) * fy) * irs;
-    const C uz = (mz + C(0.5) * fz) * irs;
-    const C usq = ux * ux + uy * uy + uz * uz;
+    C ux, uy, uz;
+    if constexpr (FORCED) {
+        ux = fma(C(0.5), fx, mx) * irs;
+        uy = fma(C(0.5), fy, my) * irs;
+        uz = fma(C(0.5), fz, mz) * irs;
+    } else {
+        ux = mx * irs;
+        uy = my * irs;
+        uz = mz * irs;
+    }
+    const C usq = fma(ux, ux, fma(uy, uy, uz * uz));
     const C om = p.omega;
     // even part without the cu^2 term: om w (drho - 1.5 rho u^2) [- 3 pref w u.F]
-    const C t = drho - C(1.5) * rho * usq;
+    const C t = fma(C(-1.5) * rho, usq, drho);
     C uf = C(0);
-    if constexpr (FORCED) uf = ux * fx + uy * fy + uz * fz;
+    if constexpr (FORCED) uf = fma(ux, fx, fma(uy, fy, uz * fz));
     C base[3], odd[3], quad[3];
 #pragma unroll
     for (int k = 1; k < 3; ++k) {
         base[k] = p.k_base[k] * t;
-        if constexpr (FORCED) base[k] -= p.k_fodd[k] * uf;
+        if constexpr (FORCED) base[k] = fma(-p.k_fodd[k], uf, base[k]);
         odd[k] = p.k_odd[k] * rho;
         quad[k] = p.k_quad[k] * rho;
     }
@@ -387,17 +427,26 @@ __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cel
 #pragma unroll
     for (int i = 1; i < Q; i += 2) {
         const int k = i <= 6 ? 1 : 2;
-        const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
-        even[i] = base[k] + quad[k] * cu * cu;
+        const C cu = cdot<C>(i, ux, uy, uz);
+        even[i] = fma(quad[k] * cu, cu, base[k]);
         oddv[i] = odd[k] * cu;
         if constexpr (FORCED) {
-            const C cf = C(CX[i]) * fx + C(CY[i]) * fy + C(CZ[i]) * fz;
-            even[i] += p.k_fquad[k] * cu * cf;
-            oddv[i] += p.k_fodd[k] * cf;
+            const C cf = cdot<C>(i, fx, fy, fz);
+            even[i] = fma(p.k_fquad[k] * cu, cf, even[i]);
+            oddv[i] = fma(p.k_fodd[k], cf, oddv[i]);
         }
     }
     // momentum residual: sum_i c_i (om feq_i + G_i) = om m + F
-    const C hx = C(0.5) * (om * mx + fx), hy = C(0.5) * (om * my + fy), hz = C(0.5) * (om * mz + fz);
+    C hx, hy, hz;
+    if constexpr (FORCED) {
+        hx = C(0.5) * fma(om, mx, fx);
+        hy = C(0.5) * fma(om, my, fy);
+        hz = C(0.5) * fma(om, mz, fz);
+    } else {
+        hx = C(0.5) * (om * mx);
+        hy = C(0.5) * (om * my);
+        hz = C(0.5) * (om * mz);
+    }
     oddv[1] = hx - (oddv[7] + oddv[9] + oddv[11] + oddv[13]);
     oddv[3] = hy - (oddv[7] - oddv[9] + oddv[15] + oddv[17]);
     oddv[5] = hz - (oddv[11] - oddv[13] + oddv[15] - oddv[17]);
@@ -405,7 +454,7 @@ __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cel
     C sum_even = C(0);
 #pragma unroll
     for (int i = 1; i < Q; i += 2) sum_even += even[i];
-    const C rest = om * drho - C(2) * sum_even;
+    const C rest = fma(C(-2), sum_even, om * drho);
     // relaxation as d + (om feq + G) - om d, one fma per population
     d[0] = fma(-om, d[0], d[0] + rest);
 #pragma unroll
@@ -419,20 +468,23 @@ __device__ __forceinline__ void collide(C d[Q], const Params<C>& p, uint32_t cel
     uz_o = uz;
 }
 
-// Read-back moments of pre-collision deviations (no collision).
+// Read-back moments of pre-collision deviations (no collision), with the
+// operations of collide(); a zero force gives the same value as the
+// unforced branch there (fma(0.5, 0, m) == m).
 template <class C>
 __device__ __forceinline__ void readback_moments(const C d[Q], const Params<C>& p, uint32_t cell,
                                                  C& rho_o, C& ux, C& uy, C& uz) {
+#pragma clang fp contract(off)
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
     C fx, fy, fz;
     cell_force(p, cell, mx, my, mz, fx, fy, fz);
     const C rs = rho > C(1e-30) ? rho : C(1e-30);
-    const C irs = C(1) / rs;  // same operation order as collide()
-    ux = (mx + C(0.5) * fx) * irs;
-    uy = (my + C(0.5) * fy) * irs;
-    uz = (mz + C(0.5) * fz) * irs;
+    const C irs = C(1) / rs;
+    ux = fma(C(0.5), fx, mx) * irs;
+    uy = fma(C(0.5), fy, my) * irs;
+    uz = fma(C(0.5), fz, mz) * irs;
     rho_o = rho;
 }
 
