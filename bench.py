@@ -277,6 +277,34 @@ def study_channel(amd, torch, shape=(256, 128, 128), tau=0.6, steps=4000, gx=1e-
     return out
 
 
+def study_mms(levels=(32, 64, 128, 256, 512, 1024), storages=("f64", "f32")):
+    """Forced Taylor-Green MMS ladder on the fused step (solution verification):
+    RMS velocity error at t_final per level and storage, observed orders."""
+    from lattice_boltzmann_d3q19_amd import verification as V
+
+    out = {"study": "mms", "workload": "forced-taylor-green-NxNx3-periodic-t0.05",
+           "known_answers_32_64_128": [V.KNOWN_ANSWERS[n] for n in (32, 64, 128)]}
+    for storage in storages:
+        runs = []
+        for n in levels:
+            r = V.mms_error_lattice(n, storage)
+            runs.append(r)
+            print(f"[study] mms {storage} N={n}: steps={r['n_steps']} err={r['err']:.4e} "
+                  f"({r['seconds']:.2f} s)", file=sys.stderr, flush=True)
+        errs = [r["err"] for r in runs]
+        out[storage] = {
+            "levels": [{k: r[k] for k in ("N", "n_steps", "tau", "ma_max", "err_u", "err_v", "err", "seconds")}
+                       for r in runs],
+            "order_pairs": [math.log(errs[i] / errs[i + 1]) / math.log(2.0) for i in range(len(errs) - 1)],
+            "order_fit_32_128": V.observed_order(levels[:3], errs[:3]),
+            "order_fit_all": V.observed_order(levels, errs),
+        }
+    if "f64" in out and "f32" in out:
+        out["f32_vs_f64_err_rel_diff"] = [abs(a["err"] - b["err"]) / b["err"]
+                                          for a, b in zip(out["f32"]["levels"], out["f64"]["levels"])]
+    return out
+
+
 def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     """The FP64 CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
     import numpy as np
@@ -284,18 +312,27 @@ def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     import oracle
 
     oracle.build() if not (ROOT / "oracle" / "_build" / "libd3q19_oracle.so").exists() else None
-    f = oracle.feq_field(rho0, u0)
-    t = time.perf_counter()
-    f = oracle.run_reference(f, tau, sample_steps)
-    dt = time.perf_counter() - t
+    f0 = oracle.feq_field(rho0, u0)
     ncell = int(np.prod(shape))
+    threads = oracle.num_threads()
+    t = time.perf_counter()
+    oracle.run_reference(f0, tau, sample_steps)
+    dt = time.perf_counter() - t
+    # the reference's own posture is single-threaded: one step on one core
+    oracle.set_num_threads(1)
+    t = time.perf_counter()
+    oracle.run_reference(f0, tau, 1)
+    dt1 = time.perf_counter() - t
+    oracle.set_num_threads(threads)
     return {
         "value": ncell * sample_steps / dt / 1e9,
         "unit": "GLUPS",
-        "cores": oracle.num_threads(),
+        "cores": threads,
         "kind": "port",
         "sample": f"{sample_steps} steps of the same {shape[0]}x{shape[1]}x{shape[2]} Taylor-Green "
                   f"lattice, FP64 C restatement of bgk_step (collide+stream), OpenMP, {dt:.2f} s",
+        "single_thread": {"value": ncell / dt1 / 1e9, "unit": "GLUPS", "cores": 1,
+                          "sample": f"1 step of the same lattice on one thread, {dt1:.2f} s"},
     }
 
 
@@ -311,7 +348,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--study", choices=("turbulence", "inverse", "channel"), default=None,
+    ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms"), default=None,
                     help="run a validation study instead of the headline benchmark")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -334,6 +371,9 @@ def main() -> int:
         return 0
     if args.study == "channel":
         print(json.dumps(study_channel(amd, torch)), flush=True)
+        return 0
+    if args.study == "mms":
+        print(json.dumps(study_mms()), flush=True)
         return 0
     n = args.size
     shape = (n, n, n)

@@ -27,7 +27,6 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
 from __future__ import annotations
 
 import ctypes
-import os
 import subprocess
 from pathlib import Path
 
@@ -72,6 +71,8 @@ def _load():
         "orc_diagnostics": (None, [i64, P, P, P, P]),
         "orc_inverse_loss": (f64, [i32, i32, i32, P, P, P, f64, i32]),
         "orc_inverse_grad": (f64, [i32, i32, i32, P, P, P, f64, i32, P, P]),
+        "orc_max_threads": (i32, []),
+        "orc_set_threads": (None, [i32]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name)
@@ -233,4 +234,9 @@ def inverse_grad(u0, F, u_tgt, tau, T):
 
 
 def num_threads() -> int:
-    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    """Threads the oracle's parallel loops use now."""
+    return int(_load().orc_max_threads())
+
+
+def set_num_threads(n: int) -> None:
+    _load().orc_set_threads(int(n))

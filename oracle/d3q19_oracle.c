@@ -37,6 +37,7 @@
  * and the FP64 discrete adjoint of the periodic Guo-forced step.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -644,3 +645,8 @@ double orc_inverse_grad(int32_t nx, int32_t ny, int32_t nz, const double* u0, co
     free(mu);
     return L;
 }
+
+/* Threads of the parallel loops (OpenMP): the current maximum, and a setter
+ * so that a caller can time the single-thread restatement. */
+int orc_max_threads(void) { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }

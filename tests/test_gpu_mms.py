@@ -1,0 +1,37 @@
+"""The MMS ladder on the engine's fused step (stateful lattice, per-step
+device-side source), against the reference's known answers
+(packages/lattice-boltzmann-d3q19/tests/test_mms_convergence.py:134-165 and
+its landing audit: 4.224e-3, 3.033e-3, 1.543e-4 at N = 32, 64, 128;
+observed order 2.39)."""
+
+from __future__ import annotations
+
+import pytest
+
+from conftest import print_case
+from mms import KNOWN_ANSWERS, mms_error_lattice, observed_order
+
+pytestmark = pytest.mark.gpu
+
+LADDER = (32, 64, 128)
+
+
+@pytest.fixture(scope="module")
+def fp64_runs():
+    out = []
+    for n in LADDER:
+        print_case("mms-lattice", storage="f64", N=n)
+        out.append(mms_error_lattice(n, "f64"))
+    return out
+
+
+def test_ladder_known_answers_fp64(fp64_runs):
+    assert [f"{r['err']:.3e}" for r in fp64_runs] == [f"{KNOWN_ANSWERS[n]:.3e}" for n in LADDER]
+    assert f"{observed_order(LADDER, [r['err'] for r in fp64_runs]):.2f}" == "2.39"
+
+
+def test_ladder_fp32_storage_tracks_fp64(fp64_runs):
+    for r64 in fp64_runs:
+        print_case("mms-lattice", storage="f32", N=r64["N"])
+        r32 = mms_error_lattice(r64["N"], "f32")
+        assert abs(r32["err"] - r64["err"]) / r64["err"] < 1e-2, (r32["err"], r64["err"])
