@@ -284,24 +284,31 @@ def study_mms(levels=(32, 64, 128, 256, 512, 1024), storages=("f64", "f32")):
 
     out = {"study": "mms", "workload": "forced-taylor-green-NxNx3-periodic-t0.05",
            "known_answers_32_64_128": [V.KNOWN_ANSWERS[n] for n in (32, 64, 128)]}
-    for storage in storages:
-        runs = []
-        for n in levels:
-            r = V.mms_error_lattice(n, storage)
-            runs.append(r)
-            print(f"[study] mms {storage} N={n}: steps={r['n_steps']} err={r['err']:.4e} "
-                  f"({r['seconds']:.2f} s)", file=sys.stderr, flush=True)
-        errs = [r["err"] for r in runs]
-        out[storage] = {
-            "levels": [{k: r[k] for k in ("N", "n_steps", "tau", "ma_max", "err_u", "err_v", "err", "seconds")}
-                       for r in runs],
-            "order_pairs": [math.log(errs[i] / errs[i + 1]) / math.log(2.0) for i in range(len(errs) - 1)],
-            "order_fit_32_128": V.observed_order(levels[:3], errs[:3]),
-            "order_fit_all": V.observed_order(levels, errs),
-        }
-    if "f64" in out and "f32" in out:
-        out["f32_vs_f64_err_rel_diff"] = [abs(a["err"] - b["err"]) / b["err"]
-                                          for a, b in zip(out["f32"]["levels"], out["f64"]["levels"])]
+    starts = {"reference_start": {}, "consistent_pressure": {"consistent_pressure": True},
+              "consistent_pressure_and_stress": {"consistent_init": True}}
+    for start, kw in starts.items():
+        for storage in storages:
+            if start == "consistent_pressure_and_stress" and storage != "f64":
+                continue
+            runs = []
+            for n in levels:
+                r = V.mms_error_lattice(n, storage, **kw)
+                runs.append(r)
+                print(f"[study] mms {start} {storage} N={n}: steps={r['n_steps']} err={r['err']:.4e} "
+                      f"({r['seconds']:.2f} s)", file=sys.stderr, flush=True)
+            errs = [r["err"] for r in runs]
+            out[f"{start}_{storage}"] = {
+                "levels": [{k: r[k] for k in ("N", "n_steps", "tau", "ma_max", "err_u", "err_v", "err", "seconds")}
+                           for r in runs],
+                "order_pairs": [math.log(errs[i] / errs[i + 1]) / math.log(2.0) for i in range(len(errs) - 1)],
+                "order_fit_32_128": V.observed_order(levels[:3], errs[:3]),
+                "order_fit_128_up": V.observed_order(levels[2:], errs[2:]),
+            }
+    for start in ("reference_start", "consistent_pressure"):
+        a, b = out.get(f"{start}_f32"), out.get(f"{start}_f64")
+        if a and b:
+            out[f"{start}_f32_vs_f64_err_rel_diff"] = [abs(x["err"] - y["err"]) / y["err"]
+                                                       for x, y in zip(a["levels"], b["levels"])]
     return out
 
 

@@ -60,8 +60,10 @@ struct d3q19_lattice {
     d3q19_config cfg;
     hipStream_t stream = nullptr;  // nullptr: the default stream
     uint32_t ncell = 0;
-    uint32_t nzp = 0;       // padded row length
-    size_t row_bytes = 0;   // 19 * nzp * esize
+    bool rows = true;       // row-blocked layout on the row grid; false: short rows
+    uint32_t nzp = 0;       // padded row length (rows), nz (short rows)
+    size_t row_bytes = 0;   // byte stride of a z-row: 19 * nzp * esize (rows), nz * esize
+    size_t dir_bytes = 0;   // byte stride of a direction: nzp * esize (rows), ncell * esize
     size_t lat_bytes = 0;   // one population buffer
     size_t esize = 0;
     void* buf[2] = {nullptr, nullptr};
@@ -89,7 +91,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.nz = (uint32_t)L->cfg.nz;
     p.ncell = L->ncell;
     p.nzp = L->nzp;
-    p.dir_bytes = (uint32_t)(L->nzp * L->esize);
+    p.dir_bytes = (uint32_t)L->dir_bytes;
     p.row_bytes = L->row_bytes;
     set_collision_constants(p, L->cfg.tau);
     p.force_mode = L->cfg.force_mode;
@@ -113,7 +115,8 @@ typename Codec<Tag>::raw* buf_of(const d3q19_lattice* L, int which) {
 }
 
 // Launch shape of every population kernel: (z, y, x) blocks whose x extent
-// is a whole number of wavefronts of one row (64, 128 or 256 cells).
+// is a whole number of wavefronts of one row (64, 128 or 256 cells); for
+// short rows a linear grid of one thread per cell.
 struct StepGrid {
     dim3 grid, block;
     uint32_t nblocks;
@@ -124,6 +127,12 @@ StepGrid step_grid(const d3q19_lattice* L) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
     const uint32_t block = step_block<Tag>();
+    if (!L->rows) {
+        sg.block = dim3(block, 1, 1);
+        sg.grid = dim3((L->ncell + block - 1) / block, 1, 1);
+        sg.nblocks = sg.grid.x;
+        return sg;
+    }
     const uint32_t bz = row_block(nz);
     const uint32_t by = block / bz;
     sg.block = dim3(bz, by, 1);
@@ -132,24 +141,24 @@ StepGrid step_grid(const d3q19_lattice* L) {
     return sg;
 }
 
-template <class Tag, bool FORCED>
+template <class Tag, bool FORCED, bool ROWS>
 void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Tag>::raw* in,
                    typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_step<Tag, BC_PERIODIC, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_PERIODIC, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             break;
         case D3Q19_BC_YWALLS:
-            k_step<Tag, BC_YWALLS, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_YWALLS, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             break;
         default:
-            k_step<Tag, BC_HALFWAY, FORCED><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_HALFWAY, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             break;
     }
 }
 
-template <class Tag>
-int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
+template <class Tag, bool ROWS>
+int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
     const StepGrid sg = step_grid<Tag>(L);
     for (int32_t s = 0; s < n; ++s) {
@@ -161,12 +170,12 @@ int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
         float* v = (s == n - 1) ? u : nullptr;
         if (L->local) {
             // first step after a load: collide f^n in place, no streaming
-            k_collide_local<Tag><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_collide_local<Tag, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             L->local = false;
         } else if (forced) {
-            launch_k_step<Tag, true>(L, sg, in, out, p, r, v);
+            launch_k_step<Tag, true, ROWS>(L, sg, in, out, p, r, v);
         } else {
-            launch_k_step<Tag, false>(L, sg, in, out, p, r, v);
+            launch_k_step<Tag, false, ROWS>(L, sg, in, out, p, r, v);
         }
         HIPCHK(hipGetLastError());
         L->cur ^= 1;
@@ -175,12 +184,20 @@ int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     return D3Q19_OK;
 }
 
+template <class Tag>
+int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
+    return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
+}
+
 template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
     const StepGrid sg = step_grid<Tag>(L);
-    k_load<Tag, In><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+    if (L->rows)
+        k_load<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+    else
+        k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
     HIPCHK(hipGetLastError());
     L->has_state = true;
     L->local = true;
@@ -188,25 +205,50 @@ int launch_init(d3q19_lattice* L, const In* f_dev) {
     return D3Q19_OK;
 }
 
+template <class Tag, class Out, bool ROWS>
+void launch_readback_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p, Out* f,
+                       Out* rho, Out* u) {
+    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_readback<Tag, BC_PERIODIC, Out, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_readback<Tag, BC_YWALLS, Out, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+        default:
+            k_readback<Tag, BC_HALFWAY, Out, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            break;
+    }
+}
+
 template <class Tag, class Out>
 int launch_readback(d3q19_lattice* L, Out* f, Out* rho, Out* u) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
     const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        launch_readback_l<Tag, Out, true>(L, sg, p, f, rho, u);
+    else
+        launch_readback_l<Tag, Out, false>(L, sg, p, f, rho, u);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+template <class Tag, bool ROWS>
+void launch_diag_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p) {
     const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_readback<Tag, BC_PERIODIC, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            k_diag<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
         case D3Q19_BC_YWALLS:
-            k_readback<Tag, BC_YWALLS, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            k_diag<Tag, BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
         default:
-            k_readback<Tag, BC_HALFWAY, Out><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
+            k_diag<Tag, BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
             break;
     }
-    HIPCHK(hipGetLastError());
-    return D3Q19_OK;
 }
 
 template <class Tag>
@@ -214,18 +256,10 @@ int launch_diag(d3q19_lattice* L) {
     typedef typename Codec<Tag>::C C;
     const Params<C> p = make_params<C>(L);
     const StepGrid sg = step_grid<Tag>(L);
-    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
-    switch (L->cfg.boundary) {
-        case D3Q19_BC_PERIODIC:
-            k_diag<Tag, BC_PERIODIC><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
-            break;
-        case D3Q19_BC_YWALLS:
-            k_diag<Tag, BC_YWALLS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
-            break;
-        default:
-            k_diag<Tag, BC_HALFWAY><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
-            break;
-    }
+    if (L->rows)
+        launch_diag_l<Tag, true>(L, sg, p);
+    else
+        launch_diag_l<Tag, false>(L, sg, p);
     HIPCHK(hipGetLastError());
     k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, sg.nblocks, NDIAG, L->dout);
     HIPCHK(hipGetLastError());
@@ -376,15 +410,28 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     L->cfg = *cfg;
     L->ncell = (uint32_t)ncell;
     L->esize = es;
-    L->nzp = padded_nz((uint32_t)cfg->nz);
-    L->row_bytes = (size_t)Q * L->nzp * es;
-    L->lat_bytes = (size_t)cfg->nx * cfg->ny * L->row_bytes;
+    // Rows shorter than a wavefront would leave most lanes idle and most of a
+    // padded row empty: such lattices are stored direction-major (the
+    // reference's layout) and stepped on a linear grid, while the 32-bit
+    // direction offsets 18 * ncell * es fit.
+    L->rows = cfg->nz >= 64 || (uint64_t)Q * ncell * es >= (1ull << 32);
+    if (L->rows) {
+        L->nzp = padded_nz((uint32_t)cfg->nz);
+        L->dir_bytes = (size_t)L->nzp * es;
+        L->row_bytes = (size_t)Q * L->nzp * es;
+        L->lat_bytes = (size_t)cfg->nx * cfg->ny * L->row_bytes;
+    } else {
+        L->nzp = (uint32_t)cfg->nz;
+        L->dir_bytes = (size_t)ncell * es;
+        L->row_bytes = (size_t)cfg->nz * es;
+        L->lat_bytes = (size_t)Q * ncell * es;
+    }
     L->stream = (hipStream_t)stream;  // NULL: the default stream
     const size_t lat_bytes = L->lat_bytes;
-    // partial sums: one slot per block of the population grid (>= ncell / 256)
-    // and per block of the linear grids
-    const uint32_t nb = (uint32_t)std::max<uint64_t>(nblocks(ncell),
-                                                     (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64));
+    // partial sums: one slot per block of the population grid and of the
+    // linear 256-thread grids (ncell / 256)
+    const uint32_t nb = (uint32_t)std::max<uint64_t>(
+        nblocks(ncell), L->rows ? (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64) : 0);
     if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
         hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {

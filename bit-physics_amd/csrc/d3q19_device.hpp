@@ -190,6 +190,28 @@ __device__ __forceinline__ bool grid_cell(uint32_t nx, uint32_t ny, uint32_t nz,
     return z < nz && y < ny && x < nx;
 }
 
+// Cell of the calling thread for a population kernel, false for padding
+// threads.  ROWS: the (z, y, x) row grid above.  Otherwise one thread per cell
+// in the reference's C order on a linear grid -- the layout of lattices whose
+// rows are shorter than a wavefront (nz < 64, e.g. the reference's depth-3
+// slabs), stored direction-major as the reference stores f:
+// row_bytes = nz * es and dir_bytes = ncell * es in the offset formula above.
+template <bool ROWS>
+__device__ __forceinline__ bool cell_of(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t& x,
+                                        uint32_t& y, uint32_t& z) {
+    if constexpr (ROWS) {
+        return grid_cell(nx, ny, nz, x, y, z);
+    } else {
+        const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+        if (c >= ncell) return false;
+        z = c % nz;
+        const uint32_t t = c / nz;
+        y = t % ny;
+        x = t / ny;
+        return true;
+    }
+}
+
 __device__ __forceinline__ uint32_t block_linear_id() {
     return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }

@@ -12,13 +12,13 @@ constexpr int BLOCK = 256;
 // Reads the post-collision state of step n-1, writes that of step n, and
 // (optionally) rho and u of the reference state f^n from the same pass.
 // ---------------------------------------------------------------------------
-template <class Tag, int BC, bool FORCED>
+template <class Tag, int BC, bool FORCED, bool ROWS = true>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
-    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
@@ -40,12 +40,12 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 
 // Load a reference-layout population field f^n as the lattice state, in
 // place (pre-collision form): stored deviations d_i = f_i - w_i.
-template <class Tag, class In>
+template <class Tag, class In, bool ROWS = true>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_load(const In* __restrict__ f, typename Codec<Tag>::raw* __restrict__ out, Params<typename Codec<Tag>::C> p) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
-    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     const bool solid = p.solid && p.solid[cell];
     C d[Q];
@@ -57,13 +57,13 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 
 // First step after a load: collide the in-place f^n (no streaming) into the
 // post-collision storage the fused step consumes; optionally write rho/u of f^n.
-template <class Tag>
+template <class Tag, bool ROWS = true>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_collide_local(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
                     Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
-    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
@@ -96,13 +96,13 @@ __device__ __forceinline__ void state_at(const void* in, const Params<typename C
 
 // Read-back: the reference's pre-collision f^n and/or its moments, in the
 // reference's (19, Nx, Ny, Nz) / (Nx, Ny, Nz) / (3, Nx, Ny, Nz) layouts.
-template <class Tag, int BC, class Out>
+template <class Tag, int BC, class Out, bool ROWS = true>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_readback(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
                Out* __restrict__ f_out, Out* __restrict__ rho_out, Out* __restrict__ u_out) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
-    if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
     const bool solid = (BC == BC_HALFWAY) && p.solid && p.solid[cell];
@@ -133,7 +133,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 // [mass, px, py, pz, kinetic energy], summed in a fixed order afterwards.
 constexpr int NDIAG = 5;
 
-template <class Tag, int BC>
+template <class Tag, int BC, bool ROWS = true>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_diag(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
            double* __restrict__ partials) {
@@ -142,7 +142,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     __shared__ double red[NW][NDIAG];
     double v[NDIAG] = {0, 0, 0, 0, 0};
     uint32_t x, y, z;
-    const bool live = grid_cell(p.nx, p.ny, p.nz, x, y, z);
+    const bool live = cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z);
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     if (live && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
         C d[Q];

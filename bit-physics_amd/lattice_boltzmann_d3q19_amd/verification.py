@@ -116,12 +116,62 @@ def mms_error(N, bgk_step, feq_field, macroscopic_velocity):
     return n_steps, tau, error_of(u, X, Y, dt, dx)[2]
 
 
-def mms_error_lattice(N, storage="f64", device=0):
+def pressure(x, y, t):
+    """Pressure of the unit-amplitude solution (solution.py:64)."""
+    return -0.25 * (np.cos(4 * np.pi * x) + np.cos(4 * np.pi * y)) * math.cos(t) ** 2
+
+
+def initial_density(X, Y, N, dt, dx):
+    """rho = 1 + p_lat / c_s^2 with the pressure of the forced flow in lattice
+    units, p_lat = p (dt / dx)^2.  The ladder scales the whole unit-amplitude
+    source by A, including its gradient part u.grad u + grad p = 2 grad p
+    (for this solution u.grad u = grad p), so the velocity A u solves the
+    forced equations with the pressure (2 A - A^2) p."""
+    p_lat = (2 * A_AMP - A_AMP * A_AMP) * pressure(X, Y, 0.0) * (dt / dx) ** 2
+    return np.repeat((1.0 + p_lat / CS2)[:, :, None], NZ, axis=2)
+
+
+def consistent_populations(X, Y, N, dt, dx, tau):
+    """f = feq(rho, u) + f1 at t = 0: the solution's density (pressure) and the
+    first-order Chapman-Enskog non-equilibrium part
+        f1_i = -tau w_i rho / c_s^2 (c_i c_i - c_s^2 I) : grad u
+    (lattice units), so that the state carries the viscous stress of the
+    solution from the first step instead of building it up."""
+    from .reference import VELOCITIES, WEIGHTS
+
+    c = np.asarray(VELOCITIES, dtype=np.float64)
+    w = np.asarray(WEIGHTS, dtype=np.float64)
+    rho = initial_density(X, Y, N, dt, dx)[:, :, 0]
+    u0, v0 = evaluate(X, Y, 0.0)
+    s = A_AMP * dt / dx
+    ux, uy = s * u0, s * v0
+    # lattice-unit gradients of (ux, uy): d/dx_lat = dx d/dx
+    tp = 2 * np.pi * dx
+    sx, cx = np.sin(2 * np.pi * X), np.cos(2 * np.pi * X)
+    sy, cy = np.sin(2 * np.pi * Y), np.cos(2 * np.pi * Y)
+    g = np.zeros((3, 3) + X.shape)  # g[a, b] = d u_b / d x_a
+    g[0, 0] = s * tp * cx * cy
+    g[1, 0] = -s * tp * sx * sy
+    g[0, 1] = s * tp * sx * sy
+    g[1, 1] = -s * tp * cx * cy
+    f = np.empty((19,) + X.shape)
+    for i in range(19):
+        cu = c[i, 0] * ux + c[i, 1] * uy
+        usq = ux * ux + uy * uy
+        feq = w[i] * rho * (1 + cu / CS2 + cu * cu / (2 * CS2 * CS2) - usq / (2 * CS2))
+        qg = sum((c[i, a] * c[i, b] - (CS2 if a == b else 0.0)) * g[a, b] for a in range(3) for b in range(3))
+        f[i] = feq - tau * w[i] * rho / CS2 * qg
+    return np.repeat(f[..., None], NZ, axis=3)
+
+
+def mms_error_lattice(N, storage="f64", device=0, consistent_pressure=False, consistent_init=False):
     """Level N on the engine's fused step; returns a dict of the run.
 
     The lattice holds a per-cell force field that is rewritten on the device
     before every step with the midpoint source; the read-back velocity uses
-    the source at t_final, as the reference's loop does.
+    the source at t_final, as the reference's loop does.  The reference
+    starts from rho = 1; ``consistent_pressure`` starts from the density of
+    the solution's pressure instead.
     """
     from . import _device as dev
     from .lattice import Lattice
@@ -152,7 +202,10 @@ def mms_error_lattice(N, storage="f64", device=0):
         F.add_(b3, alpha=c)
 
     lat = Lattice((N, N, NZ), tau, storage=storage, force=F, device=device)
-    lat.set_equilibrium(u=u0)
+    if consistent_init:
+        lat.set_populations(consistent_populations(X, Y, N, dt, dx, tau))
+    else:
+        lat.set_equilibrium(rho=initial_density(X, Y, N, dt, dx) if consistent_pressure else None, u=u0)
     torch.cuda.synchronize(dvc)
     t0 = time.perf_counter()
     for step in range(n_steps):
@@ -165,7 +218,9 @@ def mms_error_lattice(N, storage="f64", device=0):
     seconds = time.perf_counter() - t0
     lat.close()
     eu, ev, err = error_of(u, X, Y, dt, dx)
-    return {"N": N, "storage": storage, "n_steps": n_steps, "tau": tau, "ma_max": ma,
+    return {"N": N, "storage": storage, "consistent_pressure": bool(consistent_pressure),
+            "consistent_init": bool(consistent_init),
+            "n_steps": n_steps, "tau": tau, "ma_max": ma,
             "dx": dx, "err_u": eu, "err_v": ev, "err": err, "seconds": seconds}
 
 

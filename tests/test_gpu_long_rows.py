@@ -1,4 +1,4 @@
-"""GPU checks of the fused step on long rows and of its bit-level determinism.
+"""GPU checks of the fused step's two layouts and of its bit-level determinism.
 
 The collision is written with contraction off and explicit fused
 multiply-adds, so the result of a cell does not depend on the kernel it is
@@ -7,7 +7,8 @@ another kernel instantiation (a separate pull routine), so the periodic and
 the faceless half-way lattices must agree bit for bit, in every storage and
 with or without forcing.  Shapes cover rows of 128 to 1024 cells, several
 blocks per row (384 = 3 x 128, 1024 = 4 x 256) and ny not a multiple of the
-rows per block.
+rows per block; and short rows (nz < 64: direction-major storage on a linear
+grid, e.g. the reference's depth-3 slabs), where the last block is partial.
 """
 
 from __future__ import annotations
@@ -20,7 +21,8 @@ from conftest import field_scaled_err, print_case
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(3, 5, 256), (2, 3, 512), (4, 6, 128), (3, 2, 384), (2, 7, 1024)]
+SHAPES = [(3, 5, 256), (2, 3, 512), (4, 6, 128), (3, 2, 384), (2, 7, 1024),
+          (16, 8, 3), (7, 5, 9), (33, 17, 1), (5, 3, 63)]
 
 
 @pytest.fixture(scope="module")
@@ -89,3 +91,16 @@ def test_long_rows_periodic_match_oracle(amd, shape):
     lat.set_populations(f0).step(40)
     ref = oracle.run_reference(f0, 0.6, 40, force_lattice=F)
     assert field_scaled_err(lat.populations(), ref) < 1e-6
+
+
+def test_short_rows_store_no_padding(amd):
+    """A depth-3 slab holds two direction-major lattices of exactly 19 * ncell
+    populations; a 64-long row needs no padding either."""
+    print_case("short-rows-footprint")
+    for shape, es in (((256, 256, 3), 8), ((64, 32, 3), 4), ((8, 8, 64), 4)):
+        storage = "f64" if es == 8 else "f32"
+        lat = amd.Lattice(shape, 0.7, storage=storage)
+        ncell = int(np.prod(shape))
+        pops = 2 * 19 * ncell * es
+        assert pops <= lat.device_bytes < pops + 64 * 1024 + ncell * 8, (shape, lat.device_bytes)
+        lat.close()

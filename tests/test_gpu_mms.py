@@ -35,3 +35,19 @@ def test_ladder_fp32_storage_tracks_fp64(fp64_runs):
         print_case("mms-lattice", storage="f32", N=r64["N"])
         r32 = mms_error_lattice(r64["N"], "f32")
         assert abs(r32["err"] - r64["err"]) / r64["err"] < 1e-2, (r32["err"], r64["err"])
+
+
+def test_consistent_pressure_converges_at_second_order():
+    """Started from the density of the forced flow's pressure, the ladder is
+    in its asymptotic range from N = 128 (measured orders 2.10, 2.17); the
+    reference's rho = 1 start leaves an undamped acoustic transient whose
+    error (~2e-4) does not fall with N."""
+    levels = (128, 256, 512)
+    errs = []
+    for n in levels:
+        print_case("mms-lattice-consistent", storage="f64", N=n)
+        errs.append(mms_error_lattice(n, "f64", consistent_pressure=True)["err"])
+    assert observed_order(levels, errs) > 1.9, errs
+    assert errs[-1] < 5e-6
+    print_case("mms-lattice-reference-start", storage="f64", N=512)
+    assert mms_error_lattice(512, "f64")["err"] > 20 * errs[-1]

@@ -47,3 +47,27 @@ def test_ladder_levels():
         assert abs(tau - V.TAU_TARGET) < 2e-2 and abs(n * dt - V.T_FINAL) < 1e-15
     assert V.setup(32)[2] == 10 and V.setup(128)[2] == 164 and V.setup(1024)[2] == 10486
     assert abs(V.observed_order([32, 64, 128], [4.224e-3, 3.033e-3, 1.543e-4]) - 2.39) < 5e-3
+
+
+def test_scaled_source_pressure():
+    """The ladder forces A u with A S (S of the unit solution): A u then solves
+    the equations with pressure P = (2A - A^2) p, since u.grad u = grad p here."""
+    rng = np.random.default_rng(1)
+    x, y = rng.random(30), rng.random(30)
+    h, nu, A = 1e-4, V.NU_PHYS, V.A_AMP
+    P = lambda x, y, t: (2 * A - A * A) * pressure(x, y, t)  # noqa: E731
+    ev = lambda x, y, t: A * np.stack(V.evaluate(x, y, t))  # noqa: E731
+    for t in (0.0, 0.03):
+        u = ev(x, y, t)
+        ut = (ev(x, y, t + h) - ev(x, y, t - h)) / (2 * h)
+        dx = (ev(x + h, y, t) - ev(x - h, y, t)) / (2 * h)
+        dy = (ev(x, y + h, t) - ev(x, y - h, t)) / (2 * h)
+        lap = (ev(x + h, y, t) + ev(x - h, y, t) + ev(x, y + h, t) + ev(x, y - h, t) - 4 * u) / (h * h)
+        gp = np.stack([(P(x + h, y, t) - P(x - h, y, t)) / (2 * h), (P(x, y + h, t) - P(x, y - h, t)) / (2 * h)])
+        res = ut + u[0] * dx + u[1] * dy + gp - nu * lap
+        assert np.max(np.abs(res - A * np.stack(V.source(x, y, t)))) < 1e-6
+    # and the initial density carries exactly that pressure in lattice units
+    dxl, dt, n, tau, X, Y = V.setup(64)
+    rho = V.initial_density(X, Y, 64, dt, dxl)
+    want = 1 + 3 * P(X, Y, 0.0) * (dt / dxl) ** 2
+    assert np.allclose(rho[:, :, 1], want, rtol=0, atol=1e-17)
