@@ -6,13 +6,16 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 
 #include "../../include/d3q19.h"
 #include "d3q19_host.hpp"
 #include "d3q19_kernels.hpp"
+#include "d3q19_moment_codec.hpp"
 
 using namespace d3q19;
 
@@ -40,7 +43,8 @@ size_t storage_bytes(int s) {
         case D3Q19_F64: return 8;
         case D3Q19_F32: return 4;
         case D3Q19_F16:
-        case D3Q19_BF16: return 2;
+        case D3Q19_BF16:
+        case D3Q19_ME16: return 2;
     }
     return 0;
 }
@@ -76,6 +80,12 @@ struct d3q19_lattice {
     float* uscratch = nullptr;
     bool has_state = false;
     bool local = false;  // buffer cur holds pre-collision f^n in place
+    // D3Q19_ME16: buf[0] holds the moment codes, buf[1] the FP32 work lattice
+    bool me = false;
+    bool me_ranges = false;
+    double me_lo[Q] = {}, me_hi[Q] = {};
+    double* mr_partials = nullptr;  // moment-range partials and result
+    double* mr_out = nullptr;
     int64_t steps = 0;
     size_t bytes = 0;
 };
@@ -184,21 +194,107 @@ int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     return D3Q19_OK;
 }
 
+// ---- moment-encoded storage (D3Q19_ME16) -----------------------------------
+
+MeCodes make_codes(const d3q19_lattice* L) {
+    MeCodes mc;
+    std::memset(&mc, 0, sizeof(mc));
+    mc.dir_bytes = (uint32_t)(L->dir_bytes / 2);
+    mc.row_bytes = L->row_bytes / 2;
+    for (int k = 0; k < Q; ++k) {
+        double mw = 0.0;  // moment k of the rest state, (M w)_k
+        for (int i = 0; i < Q; ++i) mw += me_mono(k, i) * wgt<double>(i);
+        const double span = L->me_hi[k] - L->me_lo[k];
+        mc.lo[k] = (float)(L->me_lo[k] - mw);
+        mc.step[k] = (float)(span / 65535.0);
+        mc.scale[k] = (float)(65535.0 / span);
+    }
+    return mc;
+}
+
+float* me_work(const d3q19_lattice* L) { return (float*)L->buf[1]; }
+uint16_t* me_codes(const d3q19_lattice* L) { return (uint16_t*)L->buf[0]; }
+
+template <bool ROWS>
+int me_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
+    const StepGrid sg = step_grid<TF32>(L);
+    const MeCodes mc = make_codes(L);
+    for (int32_t s = 0; s < n; ++s) {
+        const Params<float> p = make_params<float>(L);
+        const bool forced = p.force_mode != D3Q19_FORCE_NONE || p.brink != nullptr;
+        float* r = (s == n - 1) ? rho : nullptr;
+        float* v = (s == n - 1) ? u : nullptr;
+        if (forced)
+            k_me_collide<true, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc, r, v);
+        else
+            k_me_collide<false, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc, r, v);
+        HIPCHK(hipGetLastError());
+        switch (L->cfg.boundary) {
+            case D3Q19_BC_PERIODIC:
+                k_me_stream_encode<BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L),
+                                                                                          p, mc);
+                break;
+            case D3Q19_BC_YWALLS:
+                k_me_stream_encode<BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p,
+                                                                                        mc);
+                break;
+            default:
+                k_me_stream_encode<BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L),
+                                                                                         p, mc);
+                break;
+        }
+        HIPCHK(hipGetLastError());
+        L->steps++;
+    }
+    return D3Q19_OK;
+}
+
+// Decode the codes into the work lattice as an in-place FP32 state.
+int me_decode(d3q19_lattice* L) {
+    const StepGrid sg = step_grid<TF32>(L);
+    const Params<float> p = make_params<float>(L);
+    const MeCodes mc = make_codes(L);
+    if (L->rows)
+        k_me_decode_local<true><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc);
+    else
+        k_me_decode_local<false><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
 template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
-    return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
+    if constexpr (std::is_same<Tag, TME16>::value)
+        return L->rows ? me_step_l<true>(L, n, rho, u) : me_step_l<false>(L, n, rho, u);
+    else
+        return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
 }
 
 template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
-    typedef typename Codec<Tag>::C C;
-    const Params<C> p = make_params<C>(L);
-    const StepGrid sg = step_grid<Tag>(L);
-    if (L->rows)
-        k_load<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
-    else
-        k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
-    HIPCHK(hipGetLastError());
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        if (!L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
+        const Params<float> p = make_params<float>(L);
+        const StepGrid sg = step_grid<TF32>(L);
+        const MeCodes mc = make_codes(L);
+        if (L->rows) {
+            k_load<TF32, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, me_work(L), p);
+            k_me_encode_local<true><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p, mc);
+        } else {
+            k_load<TF32, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, me_work(L), p);
+            k_me_encode_local<false><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p, mc);
+        }
+        HIPCHK(hipGetLastError());
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            k_load<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+        else
+            k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+        HIPCHK(hipGetLastError());
+    }
     L->has_state = true;
     L->local = true;
     L->steps = 0;
@@ -207,8 +303,7 @@ int launch_init(d3q19_lattice* L, const In* f_dev) {
 
 template <class Tag, class Out, bool ROWS>
 void launch_readback_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p, Out* f,
-                       Out* rho, Out* u) {
-    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
+                       Out* rho, Out* u, const typename Codec<Tag>::raw* in) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
             k_readback<Tag, BC_PERIODIC, Out, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, f, rho, u);
@@ -224,20 +319,32 @@ void launch_readback_l(d3q19_lattice* L, const StepGrid& sg, const Params<typena
 
 template <class Tag, class Out>
 int launch_readback(d3q19_lattice* L, Out* f, Out* rho, Out* u) {
-    typedef typename Codec<Tag>::C C;
-    const Params<C> p = make_params<C>(L);
-    const StepGrid sg = step_grid<Tag>(L);
-    if (L->rows)
-        launch_readback_l<Tag, Out, true>(L, sg, p, f, rho, u);
-    else
-        launch_readback_l<Tag, Out, false>(L, sg, p, f, rho, u);
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        // decode, then read the work lattice back as an in-place FP32 state
+        if (int e = me_decode(L)) return e;
+        Params<float> p = make_params<float>(L);
+        p.local = 1;
+        const StepGrid sg = step_grid<TF32>(L);
+        if (L->rows)
+            launch_readback_l<TF32, Out, true>(L, sg, p, f, rho, u, me_work(L));
+        else
+            launch_readback_l<TF32, Out, false>(L, sg, p, f, rho, u, me_work(L));
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            launch_readback_l<Tag, Out, true>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
+        else
+            launch_readback_l<Tag, Out, false>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
+    }
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
 
 template <class Tag, bool ROWS>
-void launch_diag_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p) {
-    const typename Codec<Tag>::raw* in = buf_of<Tag>(L, L->cur);
+void launch_diag_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p,
+                   const typename Codec<Tag>::raw* in) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
             k_diag<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
@@ -253,15 +360,29 @@ void launch_diag_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename C
 
 template <class Tag>
 int launch_diag(d3q19_lattice* L) {
-    typedef typename Codec<Tag>::C C;
-    const Params<C> p = make_params<C>(L);
-    const StepGrid sg = step_grid<Tag>(L);
-    if (L->rows)
-        launch_diag_l<Tag, true>(L, sg, p);
-    else
-        launch_diag_l<Tag, false>(L, sg, p);
+    uint32_t nblk;
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        if (int e = me_decode(L)) return e;
+        Params<float> p = make_params<float>(L);
+        p.local = 1;
+        const StepGrid sg = step_grid<TF32>(L);
+        if (L->rows)
+            launch_diag_l<TF32, true>(L, sg, p, me_work(L));
+        else
+            launch_diag_l<TF32, false>(L, sg, p, me_work(L));
+        nblk = sg.nblocks;
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            launch_diag_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+        else
+            launch_diag_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
+        nblk = sg.nblocks;
+    }
     HIPCHK(hipGetLastError());
-    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, sg.nblocks, NDIAG, L->dout);
+    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, nblk, NDIAG, L->dout);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -274,6 +395,7 @@ int launch_diag(d3q19_lattice* L) {
             case D3Q19_F32: return FN<TF32>(__VA_ARGS__);                       \
             case D3Q19_F16: return FN<TF16>(__VA_ARGS__);                       \
             case D3Q19_BF16: return FN<TBF16>(__VA_ARGS__);                     \
+            case D3Q19_ME16: return FN<TME16>(__VA_ARGS__);                     \
         }                                                                       \
         return fail(D3Q19_EINVAL, "bad storage type");                          \
     }()
@@ -374,6 +496,53 @@ int stage_out(const void* src, bool src_double, size_t n, void* dst, int dt, int
 
 }  // namespace d3q19
 
+namespace {
+template <class Tag, bool ROWS>
+void launch_moment_range_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p,
+                           const typename Codec<Tag>::raw* in) {
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_moment_range<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_moment_range<Tag, BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            break;
+        default:
+            k_moment_range<Tag, BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            break;
+    }
+}
+
+template <class Tag>
+int launch_moment_range(d3q19_lattice* L) {
+    uint32_t nblk;
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        if (int e = me_decode(L)) return e;
+        Params<float> p = make_params<float>(L);
+        p.local = 1;
+        const StepGrid sg = step_grid<TF32>(L);
+        if (L->rows)
+            launch_moment_range_l<TF32, true>(L, sg, p, me_work(L));
+        else
+            launch_moment_range_l<TF32, false>(L, sg, p, me_work(L));
+        nblk = sg.nblocks;
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            launch_moment_range_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+        else
+            launch_moment_range_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
+        nblk = sg.nblocks;
+    }
+    HIPCHK(hipGetLastError());
+    k_moment_range_final<<<1, 64, 0, L->stream>>>(L->mr_partials, nblk, L->mr_out);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* d3q19_last_error(void) { return g_err.c_str(); }
@@ -393,8 +562,11 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     *out = nullptr;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
     const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
-    const size_t es = storage_bytes(cfg->storage);
-    if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16 or BF16");
+    const bool me = cfg->storage == D3Q19_ME16;
+    // the moment-encoded lattice lays out its FP32 work lattice like an F32
+    // lattice and its codes in the same layout at 2 bytes per element
+    const size_t es = me ? 4 : storage_bytes(cfg->storage);
+    if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16, BF16 or ME16");
     if (ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (ncell * 8 must fit 32 bits)");
     if (cfg->nx > 65535 || cfg->ny > 65535) return fail(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (!(cfg->tau > 0.5)) return fail(D3Q19_EINVAL, "tau must be > 0.5");
@@ -410,6 +582,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     L->cfg = *cfg;
     L->ncell = (uint32_t)ncell;
     L->esize = es;
+    L->me = me;
     // Rows shorter than a wavefront would leave most lanes idle and most of a
     // padded row empty: such lattices are stored direction-major (the
     // reference's layout) and stepped on a linear grid, while the 32-bit
@@ -432,13 +605,14 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     // linear 256-thread grids (ncell / 256)
     const uint32_t nb = (uint32_t)std::max<uint64_t>(
         nblocks(ncell), L->rows ? (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64) : 0);
-    if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
+    const size_t buf0_bytes = me ? lat_bytes / 2 : lat_bytes;  // codes, or lattice 0
+    if (hipMalloc(&L->buf[0], buf0_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
         hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {
         d3q19_destroy(L);
         return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
     }
-    L->bytes = 2 * lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
+    L->bytes = buf0_bytes + lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
     *out = L;
     return D3Q19_OK;
 }
@@ -446,7 +620,8 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
 int d3q19_destroy(d3q19_lattice* L) {
     if (!L) return D3Q19_OK;
     (void)hipStreamSynchronize(L->stream);
-    void* ps[] = {L->buf[0], L->buf[1], L->force, L->brink, L->solid, L->partials, L->dout, L->uscratch};
+    void* ps[] = {L->buf[0], L->buf[1], L->force,       L->brink, L->solid, L->partials,
+                  L->dout,   L->uscratch, L->mr_partials, L->mr_out};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete L;
@@ -671,6 +846,56 @@ int d3q19_enstrophy(d3q19_lattice* L, double* out) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, L->dout, sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_moment_basis(int32_t m[361], int32_t minv4[361]) {
+    if (!m || !minv4) return fail(D3Q19_EINVAL, "null output");
+    for (int k = 0; k < Q; ++k)
+        for (int i = 0; i < Q; ++i) {
+            m[k * Q + i] = me_mono(k, i);
+            minv4[i * Q + k] = ME_MINV4[i][k];
+        }
+    return D3Q19_OK;
+}
+
+int d3q19_set_moment_ranges(d3q19_lattice* L, const double lo[19], const double hi[19]) {
+    if (int e = check_lat(L)) return e;
+    if (!L->me) return fail(D3Q19_EINVAL, "moment ranges apply to D3Q19_ME16 lattices");
+    if (!lo || !hi) return fail(D3Q19_EINVAL, "null ranges");
+    if (L->has_state) return fail(D3Q19_ESTATE, "set the moment ranges before loading a state");
+    for (int k = 0; k < Q; ++k)
+        if (!(std::isfinite(lo[k]) && std::isfinite(hi[k]) && hi[k] > lo[k]))
+            return fail(D3Q19_EINVAL, "each moment range needs finite lo < hi");
+    for (int k = 0; k < Q; ++k) {
+        L->me_lo[k] = lo[k];
+        L->me_hi[k] = hi[k];
+    }
+    L->me_ranges = true;
+    return D3Q19_OK;
+}
+
+
+int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
+    if (int e = check_lat(L)) return e;
+    if (!lo || !hi) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (!L->mr_partials) {
+        // one slot pair per block of the population grid (<= the partials of the diagnostics)
+        const size_t nb = L->rows ? (size_t)L->cfg.nx * L->cfg.ny * ((L->cfg.nz + 63) / 64) : nblocks(L->ncell);
+        if (hipMalloc((void**)&L->mr_partials, std::max<size_t>(nb, nblocks(L->ncell)) * 2 * Q * sizeof(double)) !=
+                hipSuccess ||
+            hipMalloc((void**)&L->mr_out, 2 * Q * sizeof(double)) != hipSuccess)
+            return fail(D3Q19_ENOMEM, "moment-range buffers");
+    }
+    if (int rc = D3Q19_DISPATCH(L, launch_moment_range, L)) return rc;
+    double out[2 * Q];
+    HIPCHK(hipMemcpyAsync(out, L->mr_out, sizeof(out), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    for (int k = 0; k < Q; ++k) {
+        lo[k] = out[k];
+        hi[k] = out[Q + k];
+    }
     return D3Q19_OK;
 }
 

@@ -12,11 +12,11 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libd3q19.so"
 
 D3Q19_OK, D3Q19_EINVAL, D3Q19_ENOMEM, D3Q19_EHIP, D3Q19_ESTATE = 0, 1, 2, 3, 4
-D3Q19_F64, D3Q19_F32, D3Q19_F16, D3Q19_BF16 = 0, 1, 2, 3
+D3Q19_F64, D3Q19_F32, D3Q19_F16, D3Q19_BF16, D3Q19_ME16 = 0, 1, 2, 3, 4
 BC_PERIODIC, BC_YWALLS, BC_HALFWAY = 0, 1, 2
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 
-STORAGE = {"f64": D3Q19_F64, "f32": D3Q19_F32, "f16": D3Q19_F16, "bf16": D3Q19_BF16}
+STORAGE = {"f64": D3Q19_F64, "f32": D3Q19_F32, "f16": D3Q19_F16, "bf16": D3Q19_BF16, "me16": D3Q19_ME16}
 BOUNDARY = {"periodic": BC_PERIODIC, "ywalls": BC_YWALLS, "halfway": BC_HALFWAY}
 FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5}
 
@@ -35,6 +35,7 @@ EXPORTED = (
     "d3q19_adjoint_destroy",
     "d3q19_adjoint_set_stream", "d3q19_adjoint_forward", "d3q19_adjoint_backward",
     "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
+    "d3q19_moment_basis", "d3q19_set_moment_ranges", "d3q19_moment_range",
 )
 
 
@@ -137,6 +138,9 @@ def load() -> ctypes.CDLL:
         "d3q19_adjoint_checkpoint_every": (i32, [P]),
         "d3q19_adjoint_memory_bytes": (ctypes.c_size_t, [P]),
         "d3q19_copy_kernel": (ctypes.c_int, [P, P, ctypes.c_size_t, P]),
+        "d3q19_moment_basis": (ctypes.c_int, [P, P]),
+        "d3q19_set_moment_ranges": (ctypes.c_int, [P, P, P]),
+        "d3q19_moment_range": (ctypes.c_int, [P, P, P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(lib, name, None)

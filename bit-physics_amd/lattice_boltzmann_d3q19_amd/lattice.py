@@ -59,7 +59,7 @@ class Lattice:
     ----------
     shape : (Nx, Ny, Nz)
     tau : BGK relaxation time (> 0.5)
-    storage : "f32" (default) | "f64" | "f16" | "bf16"  -- population storage;
+    storage : "f32" (default) | "f64" | "f16" | "bf16" | "me16" -- population storage;
         16-bit formats store f_i - w_i, collision stays FP32.
     boundary : "periodic" | "ywalls" (the reference's apply_bounce_back_y_walls)
         | "halfway" (textbook half-way bounce-back on `wall_faces`, solids).
@@ -279,6 +279,26 @@ class Lattice:
         out = (ctypes.c_double * 5)()
         call("d3q19_diagnostics", self._h, out)
         return {"mass": out[0], "momentum": (out[1], out[2], out[3]), "kinetic_energy": out[4]}
+
+    # ---- moment-encoded storage ----------------------------------------
+    def set_moment_ranges(self, lo, hi):
+        """Quantization ranges [lo_k, hi_k] of the 19 moments of f ("me16"
+        storage; before the first state is loaded)."""
+        lo = np.ascontiguousarray(lo, dtype=np.float64)
+        hi = np.ascontiguousarray(hi, dtype=np.float64)
+        if lo.shape != (19,) or hi.shape != (19,):
+            raise ValueError("lo and hi must hold 19 values")
+        call("d3q19_set_moment_ranges", self._h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
+        return self
+
+    def moment_range(self):
+        """(lo, hi): FP64 minimum and maximum over all cells of the 19 moments
+        m = M f of the current state."""
+        self._sync_stream()
+        lo = np.empty(19, np.float64)
+        hi = np.empty(19, np.float64)
+        call("d3q19_moment_range", self._h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
+        return lo, hi
 
     def enstrophy(self) -> float:
         self._sync_stream()

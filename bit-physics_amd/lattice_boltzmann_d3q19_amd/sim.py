@@ -8,7 +8,10 @@ packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/sim.py:
 * :func:`sim_runner_seeded_couette` -- Couette 32 x 16 x 3, top wall
   u = (0.05, 0, 0), 500 steps (sim.py:339-376,478-520);
 * :func:`sim_runner_diagnostic` -- Poiseuille 16 x 8 x 3, 50 steps, every 10th
-  captured (sim.py:523-549).
+  captured (sim.py:523-549);
+* :func:`sim_runner_moment_encoded` -- the canonical Poiseuille on 16-bit
+  moment-encoded storage, as the frontier-moment-encoded variant's capture
+  (packages/lattice-boltzmann-d3q19-frontier-moment-encoded/src/lbm_me_capture_main.cpp).
 
 Each capture frame holds rho = density_field(f^n) and
 u = macroscopic_velocity(f^n, F) in float64, plus the tier-1 health and
@@ -171,8 +174,47 @@ def sim_runner_diagnostic(seed: int, out_dir, storage: str = "f64", payload_form
                 1.0 / nx, out_dir, payload_format)
 
 
+def moment_encoded_diagnostics(rho: np.ndarray, u: np.ndarray) -> dict:
+    """Frame diagnostics of the moment-encoded variant's capture
+    (ME/src/lbm_me.cpp:465-474): density range and the largest |u| component."""
+    return {"rho_min": float(np.min(rho)), "rho_max": float(np.max(rho)), "u_max_lat": float(np.max(np.abs(u)))}
+
+
+def sim_runner_moment_encoded(seed: int, out_dir, steps: int = CANONICAL_POISEUILLE_STEPS,
+                              payload_format=None) -> Path:
+    """Canonical Poiseuille on 16-bit moment-encoded storage, every step
+    captured (ME/src/lbm_me_capture_main.cpp, lbm_me.cpp:395-486): ranges
+    calibrated over the whole horizon (margin 0.25), frame 0 is the encoded
+    initial state."""
+    from .moment_encoded import RANGE_MARGIN, WARMUP_STEPS, moment_encoded_lattice
+
+    nx, ny, nz = CANONICAL_POISEUILLE_NX, CANONICAL_POISEUILLE_NY, CANONICAL_NZ
+    t0 = time.perf_counter()
+    lat = moment_encoded_lattice((nx, ny, nz), CANONICAL_TAU, steps, boundary="ywalls",
+                                 force=(CANONICAL_POISEUILLE_FORCE_X, 0.0, 0.0))
+    params = {"tau": CANONICAL_TAU, "force_x_lattice": CANONICAL_POISEUILLE_FORCE_X, "quantize_bits": 16.0,
+              "range_margin": RANGE_MARGIN, "warmup_steps": float(WARMUP_STEPS)}
+    m = _manifest(CANONICAL_DESCRIPTOR_POISEUILLE, "test", (nx, ny, nz), seed, params, steps, 1, "f32")
+    m.sim["variant"] = "frontier-moment-encoded-16bit"
+
+    def frames():
+        for n in range(steps + 1):
+            if n:
+                lat.step(1)
+            rho, u = lat.moments()
+            yield StepState(step=n, state={"rho": rho, "u": u}, diagnostics=moment_encoded_diagnostics(rho, u))
+
+    path = write_capture(frames(), m, out_dir, payload_format)
+    lat.close()
+    mm = json.loads(path.read_text())
+    mm["run"]["wall_clock_seconds"] = time.perf_counter() - t0
+    path.write_text(json.dumps(mm, indent=2, sort_keys=True))
+    return path
+
+
 __all__ = [
     "CANONICAL_COUETTE_WALL_VELOCITY", "CANONICAL_POISEUILLE_FORCE_X", "CANONICAL_TAU",
     "DIAGNOSTIC_DESCRIPTOR", "couette_lattice", "evolve", "health_check", "poiseuille_lattice",
-    "sim_runner_diagnostic", "sim_runner_seeded", "sim_runner_seeded_couette", "vector_field_diagnostics",
+    "moment_encoded_diagnostics", "sim_runner_diagnostic", "sim_runner_moment_encoded", "sim_runner_seeded",
+    "sim_runner_seeded_couette", "vector_field_diagnostics",
 ]

@@ -12,7 +12,8 @@
  *    (3, Nx, Ny, Nz); scalar fields (Nx, Ny, Nz).
  *  - dtype codes: D3Q19_F64, D3Q19_F32 for arrays crossing the ABI; the
  *    lattice storage type may also be D3Q19_F16 / D3Q19_BF16 (populations
- *    kept on the GPU as f_i - w_i, collision in FP32).
+ *    kept on the GPU as f_i - w_i, collision in FP32) or D3Q19_ME16 (the 19
+ *    monomial moments quantized to 16 bits, collision in FP32).
  *  - on_device != 0: the pointer is device memory on the lattice's device;
  *    otherwise host memory (the library stages it).
  *  - stream: a hipStream_t passed as void*; NULL is HIP's default stream.
@@ -39,7 +40,7 @@ typedef enum d3q19_status {
     D3Q19_ESTATE = 4   /* call not valid in the lattice's current state */
 } d3q19_status;
 
-enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3 };
+enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3, D3Q19_ME16 = 4 };
 
 enum {
     D3Q19_BC_PERIODIC = 0, /* periodic on all six faces (NP/reference/bgk.py:115-126) */
@@ -115,6 +116,25 @@ int d3q19_enstrophy(d3q19_lattice* lat, double* out);
 int d3q19_synchronize(d3q19_lattice* lat);
 int64_t d3q19_step_count(const d3q19_lattice* lat);
 size_t d3q19_device_bytes(const d3q19_lattice* lat);
+
+/* ---- 16-bit moment-encoded storage (D3Q19_ME16) -------------------------
+ * Replaces the frontier-moment-encoded variant's codec
+ * (packages/lattice-boltzmann-d3q19-frontier-moment-encoded: src/lbm_me.cpp:83-150
+ * build_moment_basis, :222-253 calibrate_ranges, shaders/lbm_encode.comp,
+ * shaders/lbm_decode.comp): the lattice keeps per cell the 19 moments m = M f
+ * over the velocity-set monomials (rows 0..3: density and momentum), each as
+ * code = round((m - lo) / (hi - lo) * 65535) clamped; f = M^-1 m on use.  The
+ * codes hold the pre-collision state f^n; a step decodes, collides (FP32),
+ * pulls, applies the boundary and re-encodes.  Device memory: 38 B of codes +
+ * 76 B of FP32 work lattice per cell. */
+/* M (moments = M f, row-major 19 x 19, entries 0/+-1) and 4 M^-1 (integers). */
+int d3q19_moment_basis(int32_t m[361], int32_t minv4[361]);
+/* Quantization ranges of the 19 moments of f (not of f - w); ME16 lattices
+ * only, before the first set_populations / set_equilibrium. */
+int d3q19_set_moment_ranges(d3q19_lattice* lat, const double lo[19], const double hi[19]);
+/* FP64 minimum and maximum over all cells of the 19 moments of the current
+ * state f^n (any storage): one frame of the range calibration. */
+int d3q19_moment_range(d3q19_lattice* lat, double lo[19], double hi[19]);
 
 /* ---- stateless drop-ins (NP/reference/__init__.py:43-71) ---------------
  * Device pointers, dtype D3Q19_F64 or D3Q19_F32, (19,Nx,Ny,Nz) layout,

@@ -71,6 +71,10 @@ def _load():
         "orc_diagnostics": (None, [i64, P, P, P, P]),
         "orc_inverse_loss": (f64, [i32, i32, i32, P, P, P, f64, i32]),
         "orc_inverse_grad": (f64, [i32, i32, i32, P, P, P, f64, i32, P, P]),
+        "orc_me_basis": (f64, [P, P]),
+        "orc_me_encode": (None, [i64, P, P, P, P]),
+        "orc_me_decode": (None, [i64, P, P, P, P]),
+        "orc_run_me": (None, [i32, i32, i32, P, f64, P, i32, P, P, i32, P, P, P, P]),
         "orc_max_threads": (i32, []),
         "orc_set_threads": (None, [i32]),
     }
@@ -231,6 +235,52 @@ def inverse_grad(u0, F, u_tgt, tau, T):
     L = _load().orc_inverse_grad(nx, ny, nz, _p(u0), _p(F), _p(u_tgt), float(tau), int(T),
                                  _p(g_u0), _p(g_F))
     return L, g_u0, g_F
+
+
+def me_basis():
+    """(M, M^-1, max |M M^-1 - I|) of the moment-encoded variant's basis."""
+    m = np.empty((19, 19), np.float64)
+    minv = np.empty((19, 19), np.float64)
+    resid = _load().orc_me_basis(_p(m), _p(minv))
+    return m, minv, resid
+
+
+def me_encode(f, lo, hi):
+    """16-bit codes (19, Nx, Ny, Nz), moment-major, of populations f."""
+    f = _f64(f)
+    lo, hi = _f64(lo), _f64(hi)
+    codes = np.empty(f.shape, np.uint16)
+    _load().orc_me_encode(f[0].size, _p(f), _p(lo), _p(hi), _p(codes))
+    return codes
+
+
+def me_decode(codes, lo, hi):
+    codes = np.ascontiguousarray(codes, dtype=np.uint16)
+    lo, hi = _f64(lo), _f64(hi)
+    f = np.empty(codes.shape, np.float64)
+    _load().orc_me_decode(codes[0].size, _p(codes), _p(lo), _p(hi), _p(f))
+    return f
+
+
+def run_me(f, tau, steps, lo, hi, force_lattice=None, ywalls=False,
+           wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0)):
+    """Quantized trajectory (ME/src/lbm_me.cpp:376-393): encode, then per step
+    decode, bgk_step, y-walls, encode; returns the decoded final state."""
+    f = _f64(f).copy()
+    nx, ny, nz = f.shape[1:]
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3, nx, ny, nz)))
+    codes = np.empty(f.shape, np.uint16)
+    work = np.empty((2,) + f.shape, np.float64)
+    _load().orc_run_me(nx, ny, nz, _p(f), float(tau), _p(F), int(bool(ywalls)), _p(_f64(wall_velocity_top)),
+                       _p(_f64(wall_velocity_bottom)), int(steps), _p(_f64(lo)), _p(_f64(hi)), _p(codes), _p(work))
+    return f
+
+
+def me_moments(f):
+    """(19, Nx, Ny, Nz) moments M f of populations f (FP64, sums in basis order)."""
+    m, _, _ = me_basis()
+    f = _f64(f)
+    return np.tensordot(m, f, axes=(1, 0))
 
 
 def num_threads() -> int:

@@ -646,6 +646,128 @@ double orc_inverse_grad(int32_t nx, int32_t ny, int32_t nz, const double* u0, co
     return L;
 }
 
+/* ------------------------------------------------------------------------
+ * 16-bit moment-encoded state (frontier-moment-encoded variant,
+ * packages/lattice-boltzmann-d3q19-frontier-moment-encoded):
+ *   basis    src/lbm_me.cpp:83-90 (monomials), :104-150 (Gauss-Jordan inverse)
+ *   encode   shaders/lbm_encode.comp:37-51   m = M f, t = clamp((m-lo)/(hi-lo), 0, 1),
+ *            code = min(round(t * 65535), 65535)
+ *   decode   shaders/lbm_decode.comp:33-43   m = lo + code (hi-lo)/65535, f = M^-1 m
+ *   step     src/lbm_me.cpp:376-393          decode, collide, stream + walls, encode
+ * Codes are stored moment-major here: codes[k * ncell + cell].
+ * ---------------------------------------------------------------------- */
+static void me_monomials(int i, double* mono) {
+    const double x = CV[i][0], y = CV[i][1], z = CV[i][2];
+    const double v[Q] = {1.0,       x,         y,         z,         x * x,     y * y,     z * z,
+                         x * y,     x * z,     y * z,     x * x * y, x * x * z, y * y * x, y * y * z,
+                         z * z * x, z * z * y, x * x * y * y, x * x * z * z, y * y * z * z};
+    for (int k = 0; k < Q; ++k) mono[k] = v[k];
+}
+
+/* M (row-major, moments = M f) and its inverse by Gauss-Jordan elimination
+ * with partial pivoting; returns max |M M^-1 - I|. */
+double orc_me_basis(double* m, double* minv) {
+    double a[Q * Q];
+    for (int i = 0; i < Q; ++i) {
+        double mono[Q];
+        me_monomials(i, mono);
+        for (int k = 0; k < Q; ++k) m[k * Q + i] = mono[k];
+    }
+    memcpy(a, m, sizeof(a));
+    for (int r = 0; r < Q; ++r)
+        for (int c = 0; c < Q; ++c) minv[r * Q + c] = (r == c) ? 1.0 : 0.0;
+    for (int col = 0; col < Q; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < Q; ++r)
+            if (fabs(a[r * Q + col]) > fabs(a[piv * Q + col])) piv = r;
+        if (piv != col)
+            for (int j = 0; j < Q; ++j) {
+                double t = a[piv * Q + j];
+                a[piv * Q + j] = a[col * Q + j];
+                a[col * Q + j] = t;
+                t = minv[piv * Q + j];
+                minv[piv * Q + j] = minv[col * Q + j];
+                minv[col * Q + j] = t;
+            }
+        const double d = a[col * Q + col];
+        for (int j = 0; j < Q; ++j) {
+            a[col * Q + j] /= d;
+            minv[col * Q + j] /= d;
+        }
+        for (int r = 0; r < Q; ++r) {
+            if (r == col) continue;
+            const double f = a[r * Q + col];
+            if (f == 0.0) continue;
+            for (int j = 0; j < Q; ++j) {
+                a[r * Q + j] -= f * a[col * Q + j];
+                minv[r * Q + j] -= f * minv[col * Q + j];
+            }
+        }
+    }
+    double resid = 0.0;
+    for (int r = 0; r < Q; ++r)
+        for (int c = 0; c < Q; ++c) {
+            double s = 0.0;
+            for (int k = 0; k < Q; ++k) s += m[r * Q + k] * minv[k * Q + c];
+            const double e = fabs(s - (r == c ? 1.0 : 0.0));
+            if (e > resid) resid = e;
+        }
+    return resid;
+}
+
+void orc_me_encode(int64_t ncell, const double* f, const double* lo, const double* hi, uint16_t* codes) {
+    double m[Q * Q], minv[Q * Q];
+    orc_me_basis(m, minv);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < ncell; ++c)
+        for (int k = 0; k < Q; ++k) {
+            double mk = 0.0;
+            for (int i = 0; i < Q; ++i) mk += m[k * Q + i] * f[(int64_t)i * ncell + c];
+            double t = (mk - lo[k]) / (hi[k] - lo[k]);
+            t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+            long code = lround(t * 65535.0);
+            codes[(int64_t)k * ncell + c] = (uint16_t)(code > 65535 ? 65535 : code);
+        }
+}
+
+void orc_me_decode(int64_t ncell, const uint16_t* codes, const double* lo, const double* hi, double* f) {
+    double m[Q * Q], minv[Q * Q];
+    orc_me_basis(m, minv);
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < ncell; ++c) {
+        double mk[Q];
+        for (int k = 0; k < Q; ++k)
+            mk[k] = lo[k] + (double)codes[(int64_t)k * ncell + c] * (hi[k] - lo[k]) / 65535.0;
+        for (int i = 0; i < Q; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < Q; ++k) s += minv[i * Q + k] * mk[k];
+            f[(int64_t)i * ncell + c] = s;
+        }
+    }
+}
+
+/* The quantized trajectory: encode f, then `steps` times decode, bgk_step,
+ * (y-walls), encode; f is returned decoded.  work: 2 * 19 * ncell doubles. */
+void orc_run_me(int32_t nx, int32_t ny, int32_t nz, double* f, double tau, const double* force,
+                int32_t ywalls, const double* uw_top, const double* uw_bot, int32_t steps, const double* lo,
+                const double* hi, uint16_t* codes, double* work) {
+    const int64_t ncell = (int64_t)nx * ny * nz;
+    const int64_t n = (int64_t)Q * ncell;
+    double* a = work;
+    double* b = work + n;
+    orc_me_encode(ncell, f, lo, hi, codes);
+    for (int32_t s = 0; s < steps; ++s) {
+        orc_me_decode(ncell, codes, lo, hi, f);
+        orc_bgk_step(nx, ny, nz, f, a, tau, force, b);
+        if (ywalls)
+            orc_bounce_back_y_walls(nx, ny, nz, a, f, uw_top, uw_bot);
+        else
+            memcpy(f, a, sizeof(double) * n);
+        orc_me_encode(ncell, f, lo, hi, codes);
+    }
+    orc_me_decode(ncell, codes, lo, hi, f);
+}
+
 /* Threads of the parallel loops (OpenMP): the current maximum, and a setter
  * so that a caller can time the single-thread restatement. */
 int orc_max_threads(void) { return omp_get_max_threads(); }

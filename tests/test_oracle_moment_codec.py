@@ -1,0 +1,122 @@
+"""The oracle's restatement of the 16-bit moment codec, pinned.
+
+* the basis (ME/tests/test_lbm_me.cpp:60-71): M M^-1 = I, rows 0..3 are the
+  density and momentum moments, and 4 M^-1 is the engine's integer table;
+* the quantization round trip within the closed-form bound (hi - lo)/2/65535
+  (test_lbm_me.cpp:73-94);
+* mass within the quantization budget on a periodic quantized run
+  (test_lbm_me.cpp:115-125);
+* the published measurement of the quantized canonical Poiseuille against its
+  FP64 parent (equivalence/tolerance.toml, [defaults.lbm-quantized]: "max_rel
+  over |u| > 1e-6 cells 1.98e-2", u max_abs 3.12e-6, rho max_abs 8.5e-15 at the
+  1001-frame horizon, full-horizon ranges, margin 0.25), every frame within the
+  declared err <= 1e-5 + 5e-2 |parent|.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+
+
+def ranges_over(frames, margin=0.25):
+    lo = np.full(19, np.inf)
+    hi = np.full(19, -np.inf)
+    for f in frames:
+        m = oracle.me_moments(f).reshape(19, -1)
+        lo = np.minimum(lo, m.min(1))
+        hi = np.maximum(hi, m.max(1))
+    span = np.maximum(hi - lo, 1e-12)
+    return lo - margin * span, hi + margin * span
+
+
+def test_basis():
+    m, minv, resid = oracle.me_basis()
+    assert resid == 0.0
+    c, _, _, _ = oracle.constants()
+    assert np.array_equal(m[0], np.ones(19))
+    for a in range(3):
+        assert np.array_equal(m[1 + a], c[:, a])
+    assert np.array_equal(minv * 4, np.round(minv * 4))
+    assert np.count_nonzero(minv) == 91
+
+
+def test_engine_basis_table_matches():
+    from lattice_boltzmann_d3q19_amd._lib import load
+
+    lib = load()
+    m_eng = np.zeros(361, np.int32)
+    minv4 = np.zeros(361, np.int32)
+    assert lib.d3q19_moment_basis(ctypes.c_void_p(m_eng.ctypes.data), ctypes.c_void_p(minv4.ctypes.data)) == 0
+    m, minv, _ = oracle.me_basis()
+    assert np.array_equal(m_eng.reshape(19, 19), m)
+    assert np.array_equal(minv4.reshape(19, 19), np.round(4 * minv))
+
+
+def test_round_trip_bound():
+    rng = np.random.default_rng(42)
+    shape = (6, 5, 4)
+    f = oracle.feq_field(1 + 0.01 * rng.standard_normal(shape), 0.02 * rng.standard_normal((3,) + shape))
+    f *= 1 + 0.02 * rng.standard_normal(f.shape)
+    lo, hi = ranges_over([f], margin=0.1)
+    back = oracle.me_decode(oracle.me_encode(f, lo, hi), lo, hi)
+    err = np.abs(oracle.me_moments(back) - oracle.me_moments(f)).reshape(19, -1).max(1)
+    bound = (hi - lo) / 2 / 65535
+    assert np.all(err <= bound * (1 + 1e-9) + 1e-15 * (np.abs(hi) + np.abs(lo)))
+    # values outside the range clamp to the end codes
+    codes = oracle.me_encode(f, lo + 0.5 * (hi - lo), hi)
+    assert codes.min() == 0
+
+
+def test_periodic_quantized_mass_budget():
+    rng = np.random.default_rng(3)
+    shape = (16, 8, 3)
+    f0 = oracle.feq_field(1 + 0.01 * rng.standard_normal(shape), 0.01 * rng.standard_normal((3,) + shape))
+    frames, f = [f0], f0
+    for _ in range(24):
+        f = oracle.bgk_step(f, 0.8)
+        frames.append(f)
+    lo, hi = ranges_over(frames)
+    q = oracle.run_me(f0, 0.8, 24, lo, hi)
+    q0 = oracle.me_decode(oracle.me_encode(f0, lo, hi), lo, hi)
+    budget = 25 * np.prod(shape) * (hi[0] - lo[0]) / 2 / 65535
+    assert abs(q.sum() - q0.sum()) <= budget
+    assert np.abs(q - frames[-1]).max() > 0  # genuinely quantized
+
+
+def test_canonical_poiseuille_reproduces_published_measurement():
+    threads = oracle.num_threads()
+    oracle.set_num_threads(1)  # small lattice: one thread is fastest
+    try:
+        shape, tau, steps = (64, 32, 3), 0.7, 1000
+        F = np.zeros((3,) + shape)
+        F[0] = 1e-5
+        f = oracle.feq_field(np.ones(shape), np.zeros((3,) + shape))
+        parent = [f]
+        for _ in range(steps):
+            parent.append(oracle.run_reference(parent[-1], tau, 1, force_lattice=F, ywalls=True))
+        lo, hi = ranges_over(parent)
+        q = oracle.me_decode(oracle.me_encode(f, lo, hi), lo, hi)
+        rho_abs = u_abs = max_rel = 0.0
+        for n in range(steps + 1):
+            if n:
+                q = oracle.run_me(q, tau, 1, lo, hi, force_lattice=F, ywalls=True)
+            rp, up = oracle.density_field(parent[n]), oracle.macroscopic_velocity(parent[n], F)
+            rq, uq = oracle.density_field(q), oracle.macroscopic_velocity(q, F)
+            e = np.abs(uq - up)
+            assert np.all(e <= 1e-5 + 5e-2 * np.abs(up)), n
+            assert np.all(np.abs(rq - rp) <= 1e-5 + 5e-2 * np.abs(rp)), n
+            rho_abs = max(rho_abs, float(np.abs(rq - rp).max()))
+            u_abs = max(u_abs, float(e.max()))
+            mask = np.abs(up) > 1e-6
+            if mask.any():
+                max_rel = max(max_rel, float((e[mask] / np.abs(up[mask])).max()))
+    finally:
+        oracle.set_num_threads(threads)
+    assert max_rel == pytest.approx(1.98e-2, abs=5e-4)
+    assert 1e-6 < u_abs < 3.2e-6
+    assert rho_abs < 5e-14
