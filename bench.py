@@ -1,7 +1,7 @@
 """Benchmark: D3Q19 GLUPS at 256^3 (Taylor-Green, periodic, tau 0.55, u0 0.04),
 FP32 storage vs FP16 (f_i - w_i) storage, % of HBM roofline.
 
-    python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-fp16]
+    python bench.py [--steps K] [--warmup W] [--size N] [--no-cpu] [--no-fp16] [--no-me16]
 
 A "step" is one lattice time step over the whole 256^3 lattice (one launch of
 the fused stream-collide kernel).  Populations start resident in HBM (set
@@ -26,14 +26,24 @@ Q = 19
 
 
 def bytes_per_cell(storage: str) -> int:
-    """Algorithmic HBM bytes per cell update: read 19 + write 19 populations."""
+    """Algorithmic HBM bytes per cell update: read 19 + write 19 populations.
+    The moment-encoded step makes two passes: codes in (38 B) + FP32 work out
+    (76 B), then work in (76 B) + codes out (38 B)."""
+    if storage == "me16":
+        return 2 * Q * (2 + 4)
     return 2 * Q * {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[storage]
 
 
 def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup):
     stream = torch.cuda.current_stream()
-    lat = amd.Lattice(shape, tau, storage=storage)
-    lat.set_equilibrium(rho0, u0)
+    if storage == "me16":
+        # ranges of the moments over the whole run (warm-up + timed steps)
+        from lattice_boltzmann_d3q19_amd.moment_encoded import moment_encoded_lattice
+
+        lat = moment_encoded_lattice(shape, tau, warmup + steps, rho=rho0, u=u0)
+    else:
+        lat = amd.Lattice(shape, tau, storage=storage)
+        lat.set_equilibrium(rho0, u0)
     lat.step(warmup)
     torch.cuda.synchronize()
     t0 = torch.cuda.Event(enable_timing=True)
@@ -119,9 +129,14 @@ def study_turbulence(amd, torch, n=384, steps=4000, every=250, tau=0.51, rms=0.0
     u0 = random_solenoidal(shape, rms, seed=seed)
     div = spectral_divergence(u0)
     runs = {}
-    for storage in ("f64", "f32", "f16", "bf16"):
-        lat = amd.Lattice(shape, tau, storage=storage)
-        lat.set_equilibrium(None, u0)
+    for storage in ("f64", "f32", "f16", "bf16", "me16"):
+        if storage == "me16":
+            from lattice_boltzmann_d3q19_amd.moment_encoded import moment_encoded_lattice
+
+            lat = moment_encoded_lattice(shape, tau, steps, u=u0)
+        else:
+            lat = amd.Lattice(shape, tau, storage=storage)
+            lat.set_equilibrium(None, u0)
         series = []
         total_ms = 0.0
         stream = torch.cuda.current_stream()
@@ -143,7 +158,7 @@ def study_turbulence(amd, torch, n=384, steps=4000, every=250, tau=0.51, rms=0.0
     out = {"study": "turbulence", "workload": f"decaying-turbulence-{n}^3-periodic-tau{tau}-{steps}steps",
            "ic": {"rms": rms, "spectrum": "k^4 exp(-2 (k/8)^2)", "seed": seed, "spectral_divergence": div},
            "reference": "FP64 storage (FP64 collision) on the GPU"}
-    for storage in ("f32", "f16", "bf16"):
+    for storage in ("f32", "f16", "bf16", "me16"):
         ser = runs[storage]["series"]
         ke = max(abs(a[1] - b[1]) / b[1] for a, b in zip(ser, ref))
         en = max(abs(a[2] - b[2]) / b[2] for a, b in zip(ser, ref))
@@ -354,6 +369,7 @@ def main() -> int:
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
+    ap.add_argument("--no-me16", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms"), default=None,
                     help="run a validation study instead of the headline benchmark")
@@ -451,6 +467,20 @@ def main() -> int:
             "achieved_GBs": round(ach16, 1),
             "frac": round(ach16 / HBM_PEAK_GBS, 4),
             "ke_rel_diff_vs_fp32": abs(diag16["kinetic_energy"] - diag["kinetic_energy"]) / diag["kinetic_energy"],
+        }
+
+    if not args.no_me16:
+        ms_me, _, diag_me = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps, args.warmup)
+        glups_me = ncell * args.steps / (ms_me * 1e-3) / 1e9
+        ach_me = bytes_per_cell("me16") * ncell / (ms_me / args.steps * 1e-3) / 1e9
+        result["me16_store"] = {
+            "value": round(glups_me, 4),
+            "unit": "GLUPS",
+            "ratio_vs_fp32": round(glups_me / glups, 4),
+            "algorithmic_bytes_per_cell": bytes_per_cell("me16"),
+            "achieved_GBs": round(ach_me, 1),
+            "frac": round(ach_me / HBM_PEAK_GBS, 4),
+            "ke_rel_diff_vs_fp32": abs(diag_me["kinetic_energy"] - diag["kinetic_energy"]) / diag["kinetic_energy"],
         }
 
     if not args.no_adjoint:

@@ -537,7 +537,7 @@ int launch_moment_range(d3q19_lattice* L) {
         nblk = sg.nblocks;
     }
     HIPCHK(hipGetLastError());
-    k_moment_range_final<<<1, 64, 0, L->stream>>>(L->mr_partials, nblk, L->mr_out);
+    k_moment_range_final<<<2 * Q, 256, 0, L->stream>>>(L->mr_partials, nblk, L->mr_out);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }

@@ -290,16 +290,29 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     }
 }
 
-static __global__ void __launch_bounds__(64)
+// Final reduction of the per-block partials: one 256-thread block per slot
+// (38 blocks: 19 minima, 19 maxima).
+static __global__ void __launch_bounds__(256)
     k_moment_range_final(const double* __restrict__ partials, uint32_t nblocks, double* __restrict__ out) {
-    const int t = threadIdx.x;
-    if (t >= 2 * Q) return;
-    double v = t < Q ? 1e300 : -1e300;
-    for (uint32_t b = 0; b < nblocks; ++b) {
+    __shared__ double red[4];
+    const int t = blockIdx.x;  // slot: 0..18 minima, 19..37 maxima
+    const bool is_min = t < Q;
+    double v = is_min ? 1e300 : -1e300;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
         const double p = partials[(size_t)b * 2 * Q + t];
-        v = t < Q ? fmin(v, p) : fmax(v, p);
+        v = is_min ? fmin(v, p) : fmax(v, p);
     }
-    out[t] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double w = __shfl_xor(v, o, 64);
+        v = is_min ? fmin(v, w) : fmax(v, w);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) v = is_min ? fmin(v, red[w]) : fmax(v, red[w]);
+        out[t] = v;
+    }
 }
 
 }  // namespace d3q19

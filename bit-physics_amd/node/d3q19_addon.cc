@@ -10,6 +10,8 @@
 //   lat.readPopulations("f64" | "f32");  // Float64Array | Float32Array, (19,Nx,Ny,Nz)
 //   lat.readMoments();                   // {rho: Float64Array, u: Float64Array}
 //   lat.diagnostics();                   // {mass, momentum: [3], kineticEnergy}
+//   lat.setMomentRanges(lo, hi);         // "me16" storage: moment quantization ranges
+//   lat.momentRange();                   // {lo, hi} of the 19 moments of the state
 //   lat.close();
 //
 // All arrays are host memory; the library stages them to the GPU.
@@ -146,7 +148,8 @@ napi_value Construct(napi_env env, napi_callback_info info) {
         else if (s == "f32") cfg.storage = D3Q19_F32;
         else if (s == "f16") cfg.storage = D3Q19_F16;
         else if (s == "bf16") cfg.storage = D3Q19_BF16;
-        else { napi_throw_range_error(env, nullptr, "storage must be f64|f32|f16|bf16"); return nullptr; }
+        else if (s == "me16") cfg.storage = D3Q19_ME16;
+        else { napi_throw_range_error(env, nullptr, "storage must be f64|f32|f16|bf16|me16"); return nullptr; }
     }
     if (get_prop_string(env, argv[0], "boundary", &s)) {
         if (s == "periodic") cfg.boundary = D3Q19_BC_PERIODIC;
@@ -296,6 +299,43 @@ napi_value Diagnostics(napi_env env, napi_callback_info info) {
     return obj;
 }
 
+// setMomentRanges(lo, hi): Float64Array(19) each, "me16" lattices before a state is loaded.
+napi_value SetMomentRanges(napi_env env, napi_callback_info info) {
+    size_t argc = 2;
+    napi_value argv[2];
+    Wrapped* w = unwrap(env, info, &argc, argv);
+    if (!w) return nullptr;
+    void *lo = nullptr, *hi = nullptr;
+    int dl = -1, dh = -1;
+    size_t nl = 0, nh = 0;
+    if (argc < 2 || !typed_array(env, argv[0], &lo, &dl, &nl) || !typed_array(env, argv[1], &hi, &dh, &nh) ||
+        dl != D3Q19_F64 || dh != D3Q19_F64 || nl != 19 || nh != 19) {
+        napi_throw_type_error(env, nullptr, "setMomentRanges(lo, hi): two Float64Array of 19 values");
+        return nullptr;
+    }
+    int st = d3q19_set_moment_ranges(w->lat, (const double*)lo, (const double*)hi);
+    if (st) return throw_status(env, "d3q19_set_moment_ranges", st);
+    return nullptr;
+}
+
+// momentRange(): {lo, hi} of the 19 moments of the current state.
+napi_value MomentRange(napi_env env, napi_callback_info info) {
+    size_t argc = 0;
+    Wrapped* w = unwrap(env, info, &argc, nullptr);
+    if (!w) return nullptr;
+    void *plo = nullptr, *phi = nullptr;
+    napi_value lo = make_typed(env, D3Q19_F64, 19, &plo);
+    napi_value hi = make_typed(env, D3Q19_F64, 19, &phi);
+    if (!lo || !hi) return nullptr;
+    int st = d3q19_moment_range(w->lat, (double*)plo, (double*)phi);
+    if (st) return throw_status(env, "d3q19_moment_range", st);
+    napi_value obj;
+    NAPI_CALL(env, napi_create_object(env, &obj));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "lo", lo));
+    NAPI_CALL(env, napi_set_named_property(env, obj, "hi", hi));
+    return obj;
+}
+
 napi_value StepCount(napi_env env, napi_callback_info info) {
     size_t argc = 0;
     Wrapped* w = unwrap(env, info, &argc, nullptr);
@@ -333,6 +373,8 @@ napi_value Init(napi_env env, napi_value exports) {
         {"readMoments", nullptr, ReadMoments, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"diagnostics", nullptr, Diagnostics, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"stepCount", nullptr, StepCount, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"setMomentRanges", nullptr, SetMomentRanges, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"momentRange", nullptr, MomentRange, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"close", nullptr, Close, nullptr, nullptr, nullptr, napi_default, nullptr},
     };
     napi_value cls;

@@ -64,4 +64,34 @@ const out = { version: version() };
   assert(threw, 'negative step count must throw');
   lat.close();
 }
+{
+  // moment-encoded storage: ranges from the FP64 trajectory, then the quantized run
+  const nx = 64, ny = 32, nz = 3, n = nx * ny * nz, steps = 1000;
+  const cfg = { nx, ny, nz, tau: 0.7, boundary: 'ywalls', force: [1e-5, 0, 0] };
+  const ref = new Lattice({ ...cfg, storage: 'f64' });
+  ref.setEquilibrium(null, new Float64Array(3 * n));
+  let { lo, hi } = ref.momentRange();
+  for (let s = 0; s < steps; ++s) {
+    ref.step(1);
+    const r = ref.momentRange();
+    for (let k = 0; k < 19; ++k) { lo[k] = Math.min(lo[k], r.lo[k]); hi[k] = Math.max(hi[k], r.hi[k]); }
+  }
+  const uRef = ref.readMoments().u;
+  ref.close();
+  for (let k = 0; k < 19; ++k) {
+    const span = Math.max(hi[k] - lo[k], 1e-12);
+    lo[k] -= 0.25 * span; hi[k] += 0.25 * span;
+  }
+  const lat = new Lattice({ ...cfg, storage: 'me16' });
+  lat.setMomentRanges(lo, hi);
+  lat.setEquilibrium(null, new Float64Array(3 * n));
+  lat.step(steps);
+  const u = lat.readMoments().u;
+  let err = 0;
+  for (let i = 0; i < u.length; ++i) err = Math.max(err, Math.abs(u[i] - uRef[i]) - 5e-2 * Math.abs(uRef[i]));
+  out.me16_poiseuille_umax = maxSpeed(u, n).toExponential(2);
+  assert(err <= 1e-5, 'me16 within the lbm-quantized tolerance of the FP64 run');
+  assert.strictEqual(out.me16_poiseuille_umax, '8.65e-3');
+  lat.close();
+}
 console.log(JSON.stringify(out));

@@ -26,3 +26,4 @@ def test_node_addon_known_answers():
     out = json.loads(r.stdout.strip().splitlines()[-1])
     assert out["poiseuille_umax"] == "8.6508320e-3"
     assert out["couette_umax"] == "5.0000000e-2"
+    assert out["me16_poiseuille_umax"] == "8.65e-3"
