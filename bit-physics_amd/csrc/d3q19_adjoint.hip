@@ -85,7 +85,7 @@ __global__ void __launch_bounds__(step_block<Tag>())
     if (!grid_cell(p.nx, p.ny, p.nz, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     C d[Q];
-    pull<Tag, BC>(H, p, x, y, z, d);
+    pull<Tag, BC, true>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
@@ -129,12 +129,12 @@ __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         if (BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p))
-            mu[i] = ld<Tag>(self_row, OPP[i], p.dir_bytes, z * ES);
+            mu[i] = ld<Tag, true>(self_row, OPP[i], p.dir_bytes, z * ES);
         else
-            mu[i] = ld<Tag>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
+            mu[i] = ld<Tag, true>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
     }
     C d[Q];
-    pull<Tag, BC>(H, p, x, y, z, d);
+    pull<Tag, BC, true>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;

@@ -221,19 +221,24 @@ __device__ __forceinline__ const char* row_of(const void* base, const Params<C>&
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
 }
 
-// Population i of a row at byte offset zoff.  Pulled values are read exactly
-// once per step: 4-byte populations load with the nontemporal hint (measured
-// faster at 256^3); 2- and 8-byte ones load normally (measured slower with it).
-template <class Tag>
+// Population i of a row at byte offset zoff.  The fused step uses plain
+// loads: the lines that a z-shifted pull straddles are shared with the
+// neighbouring wavefront and stay in L2 (with the nontemporal hint they were
+// re-fetched: 12% more FP32 reads, measured with FETCH_SIZE at 256^3).  The
+// adjoint's 4-byte loads keep the hint (NT): its backward pass measured 4%
+// faster with it.
+template <class Tag, bool NT = false>
 __device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
     typedef typename Codec<Tag>::raw R;
     const R* a = (const R*)(row + (uint32_t)i * dir_bytes + zoff);
-    if constexpr (sizeof(R) == 4)
+    if constexpr (NT && sizeof(R) == 4)
         return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
     else
         return Codec<Tag>::dec(*a, i);
 }
 
+// Stores are nontemporal: each population is written once per step and read
+// in the next (plain FP32 stores measured 12% slower at 256^3).
 template <class Tag>
 __device__ __forceinline__ void st(char* row, int i, uint32_t dir_bytes, uint32_t zoff, typename Codec<Tag>::C v) {
     typedef typename Codec<Tag>::raw R;
@@ -303,7 +308,7 @@ __device__ __forceinline__ void ywall_swap(const Params<C>& p, uint32_t y, C d[Q
 // bounce-back, from the stored post-collision deviations.  Direction i pulls
 // from row (x - cx, y - cy) at z - cz.
 // ---------------------------------------------------------------------------
-template <class Tag, int BC>
+template <class Tag, int BC, bool NT = false>
 __device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                      uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
 #pragma clang fp contract(off)
@@ -318,7 +323,7 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
-            d[i] = ld<Tag>(row, i, DB, zs[1 - CZ[i]] * ES);
+            d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
         }
     }
     if constexpr (BC == BC_YWALLS) ywall_swap<C>(p, y, d);
@@ -336,13 +341,13 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
             if (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u)) face = 5;
             if (face >= 0) {
                 const C cu = cdot<C>(i, p.face_u[face][0], p.face_u[face][1], p.face_u[face][2]);
-                d[i] = ld<Tag>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;
+                d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;
             } else {
                 const uint32_t sx = xs[1 - CX[i]], sy = ys[1 - CY[i]], sz = zs[1 - CZ[i]];
                 if (p.solid && p.solid[(sx * p.ny + sy) * p.nz + sz])
-                    d[i] = ld<Tag>(self, OPP[i], DB, z * ES);
+                    d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES);
                 else
-                    d[i] = ld<Tag>(row_of(in, p, sx, sy), i, DB, sz * ES);
+                    d[i] = ld<Tag, NT>(row_of(in, p, sx, sy), i, DB, sz * ES);
             }
         }
     }
