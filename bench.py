@@ -31,7 +31,7 @@ def bytes_per_cell(storage: str) -> int:
     (76 B), then work in (76 B) + codes out (38 B)."""
     if storage == "me16":
         return 2 * Q * (2 + 4)
-    return 2 * Q * {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}[storage]
+    return 2 * Q * {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "bf16sr": 2}[storage]
 
 
 def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup):
@@ -129,7 +129,7 @@ def study_turbulence(amd, torch, n=384, steps=4000, every=250, tau=0.51, rms=0.0
     u0 = random_solenoidal(shape, rms, seed=seed)
     div = spectral_divergence(u0)
     runs = {}
-    for storage in ("f64", "f32", "f16", "bf16", "me16"):
+    for storage in ("f64", "f32", "f16", "bf16", "bf16sr", "me16"):
         if storage == "me16":
             from lattice_boltzmann_d3q19_amd.moment_encoded import moment_encoded_lattice
 
@@ -158,7 +158,7 @@ def study_turbulence(amd, torch, n=384, steps=4000, every=250, tau=0.51, rms=0.0
     out = {"study": "turbulence", "workload": f"decaying-turbulence-{n}^3-periodic-tau{tau}-{steps}steps",
            "ic": {"rms": rms, "spectrum": "k^4 exp(-2 (k/8)^2)", "seed": seed, "spectral_divergence": div},
            "reference": "FP64 storage (FP64 collision) on the GPU"}
-    for storage in ("f32", "f16", "bf16", "me16"):
+    for storage in ("f32", "f16", "bf16", "bf16sr", "me16"):
         ser = runs[storage]["series"]
         ke = max(abs(a[1] - b[1]) / b[1] for a, b in zip(ser, ref))
         en = max(abs(a[2] - b[2]) / b[2] for a, b in zip(ser, ref))

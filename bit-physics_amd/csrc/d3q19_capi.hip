@@ -44,6 +44,7 @@ size_t storage_bytes(int s) {
         case D3Q19_F32: return 4;
         case D3Q19_F16:
         case D3Q19_BF16:
+        case D3Q19_BF16SR:
         case D3Q19_ME16: return 2;
     }
     return 0;
@@ -116,6 +117,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.brink = (const C*)L->brink;
     p.solid = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->solid : nullptr;
     p.local = L->local ? 1 : 0;
+    p.seed = (uint32_t)L->steps * 0x9e3779b9u + 0x7f4a7c15u;
     return p;
 }
 
@@ -395,6 +397,7 @@ int launch_diag(d3q19_lattice* L) {
             case D3Q19_F32: return FN<TF32>(__VA_ARGS__);                       \
             case D3Q19_F16: return FN<TF16>(__VA_ARGS__);                       \
             case D3Q19_BF16: return FN<TBF16>(__VA_ARGS__);                     \
+            case D3Q19_BF16SR: return FN<TBF16S>(__VA_ARGS__);                  \
             case D3Q19_ME16: return FN<TME16>(__VA_ARGS__);                     \
         }                                                                       \
         return fail(D3Q19_EINVAL, "bad storage type");                          \
@@ -566,7 +569,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     // the moment-encoded lattice lays out its FP32 work lattice like an F32
     // lattice and its codes in the same layout at 2 bytes per element
     const size_t es = me ? 4 : storage_bytes(cfg->storage);
-    if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16, BF16 or ME16");
+    if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16, BF16, BF16SR or ME16");
     if (ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (ncell * 8 must fit 32 bits)");
     if (cfg->nx > 65535 || cfg->ny > 65535) return fail(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (!(cfg->tau > 0.5)) return fail(D3Q19_EINVAL, "tau must be > 0.5");

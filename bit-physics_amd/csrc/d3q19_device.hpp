@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace d3q19 {
 
 constexpr int Q = 19;
@@ -55,7 +57,8 @@ enum { FORCE_NONE = 0, FORCE_UNIFORM = 1, FORCE_FIELD = 2 };
 struct TF64 {};
 struct TF32 {};
 struct TF16 {};   // IEEE half of (f_i - w_i) * (2048 / w_i)
-struct TBF16 {};  // bfloat16 of (f_i - w_i)
+struct TBF16 {};  // bfloat16 of (f_i - w_i), rounded to nearest even
+struct TBF16S {}; // bfloat16 of (f_i - w_i), stochastically rounded
 
 template <class Tag> struct Codec;
 
@@ -103,7 +106,18 @@ template <> struct Codec<TBF16> {
         b += 0x7fffu + ((b >> 16) & 1u);  // round to nearest even
         return (raw)(b >> 16);
     }
+    // Stochastic rounding with a uniform 16-bit r: rounds the magnitude up
+    // with probability (dropped bits) / 2^16, so the stored value is unbiased.
+    // Branch-free: Inf has zero low bits and stays Inf; NaN becomes the quiet NaN.
+    __device__ static __forceinline__ raw enc_sr(float d, uint32_t r) {
+        const uint32_t b = __builtin_bit_cast(uint32_t, d);
+        const uint32_t v = (b + (r & 0xffffu)) >> 16;
+        return (raw)(d != d ? 0x7fc0u : v);
+    }
 };
+
+// Same format as TBF16; stores round stochastically (store_local).
+template <> struct Codec<TBF16S> : Codec<TBF16> {};
 
 // Threads per block of the population kernels: one 1 KiB row segment per
 // direction and block (256 cells of 4 or 8 bytes, 512 cells of 2 bytes).
@@ -140,6 +154,7 @@ template <class C> struct Params {
     C face_u[6][3];         // half-way wall velocities
     C wtop[3], wbot[3];     // reference y-wall velocities
     int32_t local;          // state holds pre-collision f^n in place (not yet streamed)
+    uint32_t seed;          // per-step key of the BF16 stochastic rounding
     // collision coefficients per weight class k (rest, face, edge), each the
     // correctly rounded value of its exact product: om w_k, 3 om w_k,
     // 4.5 om w_k, 3 pref w_k, 9 pref w_k
@@ -255,13 +270,40 @@ __device__ __forceinline__ void load_local(const void* in, const Params<typename
     for (int i = 0; i < Q; ++i) d[i] = ld<Tag>(row, i, p.dir_bytes, z * ES);
 }
 
+// 32-bit integer hash (two xorshift-multiply rounds).
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
 template <class Tag>
 __device__ __forceinline__ void store_local(void* out, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                             uint32_t y, uint32_t z, const typename Codec<Tag>::C d[Q]) {
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     char* row = (char*)row_of(out, p, x, y);
+    if constexpr (std::is_same<Tag, TBF16S>::value) {
+        // BF16 keeps 8 mantissa bits: nearest rounding of the small deviations
+        // biases them the same way every step (kinetic energy of decaying
+        // turbulence off by 1.4e-2 at 384^3); stochastic rounding removes the
+        // bias.  The rounding bits come from a hash of (cell, step) and one
+        // multiply per population, so a run is reproducible bit for bit.
+        const uint32_t key = mix32(((x * p.ny + y) * p.nz + z) ^ p.seed);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) st<Tag>(row, i, p.dir_bytes, z * ES, d[i]);
+        for (int i = 0; i < Q; ++i) {
+            // key * odd constant (mod 2^32) is a bijection of the hashed key,
+            // so its top 16 bits are uniform
+            const uint32_t r = (key * (0x9e3779b1u * (uint32_t)(2 * i + 1))) >> 16;
+            __builtin_nontemporal_store(Codec<TBF16>::enc_sr(d[i], r),
+                                        (uint16_t*)(row + (uint32_t)i * p.dir_bytes + z * ES));
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) st<Tag>(row, i, p.dir_bytes, z * ES, d[i]);
+    }
 }
 
 // Reference y-wall treatment (reference/bgk.py:190-215) applied in registers

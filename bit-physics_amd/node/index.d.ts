@@ -2,7 +2,7 @@
 // Populations are (19, Nx, Ny, Nz) z-fastest in the reference's lexicographic
 // velocity order; vector fields are (3, Nx, Ny, Nz).
 
-export type Storage = 'f64' | 'f32' | 'f16' | 'bf16' | 'me16';
+export type Storage = 'f64' | 'f32' | 'f16' | 'bf16' | 'bf16sr' | 'me16';
 export type Boundary = 'periodic' | 'ywalls' | 'halfway';
 export type Face = '-x' | '+x' | '-y' | '+y' | '-z' | '+z';
 export type Vec3 = [number, number, number];

@@ -11,9 +11,9 @@
  *    (NP/reference/constants.py:39-59).  Vector fields (velocity, force) are
  *    (3, Nx, Ny, Nz); scalar fields (Nx, Ny, Nz).
  *  - dtype codes: D3Q19_F64, D3Q19_F32 for arrays crossing the ABI; the
- *    lattice storage type may also be D3Q19_F16 / D3Q19_BF16 (populations
- *    kept on the GPU as f_i - w_i, collision in FP32) or D3Q19_ME16 (the 19
- *    monomial moments quantized to 16 bits, collision in FP32).
+ *    lattice storage type may also be D3Q19_F16 / D3Q19_BF16 / D3Q19_BF16SR
+ *    (populations kept on the GPU as f_i - w_i, collision in FP32) or
+ *    D3Q19_ME16 (the 19 monomial moments quantized to 16 bits, collision in FP32).
  *  - on_device != 0: the pointer is device memory on the lattice's device;
  *    otherwise host memory (the library stages it).
  *  - stream: a hipStream_t passed as void*; NULL is HIP's default stream.
@@ -40,7 +40,9 @@ typedef enum d3q19_status {
     D3Q19_ESTATE = 4   /* call not valid in the lattice's current state */
 } d3q19_status;
 
-enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3, D3Q19_ME16 = 4 };
+/* Storage D3Q19_BF16SR is D3Q19_BF16 with stochastic rounding of every store
+ * (unbiased; the rounding bits are a hash of cell and step, so runs repeat). */
+enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3, D3Q19_ME16 = 4, D3Q19_BF16SR = 5 };
 
 enum {
     D3Q19_BC_PERIODIC = 0, /* periodic on all six faces (NP/reference/bgk.py:115-126) */

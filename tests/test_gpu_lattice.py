@@ -97,15 +97,18 @@ def test_ywalls_long_run_matches_oracle(amd, storage):
     assert field_scaled_err(lat.populations(), ref) < TOL[storage]
 
 
-# 16-bit storage keeps f - w in FP16 (scaled by 2048 / w_i) or BF16 and
-# collides in FP32.  Measured against the FP64 oracle after 200 steps:
-# deviation error (max |d - d_ref| / max |d_ref|) FP16 1.0e-3, BF16 8.1e-3;
-# KE relative error FP16 3.7e-5, BF16 1.1e-3.  The bounds carry ~3x headroom.
+# 16-bit storage keeps f - w in FP16 (scaled by 2048 / w_i) or BF16 (nearest
+# or stochastic rounding) and collides in FP32.  Measured against the FP64
+# oracle after 200 steps (TG / y-walls): deviation error
+# (max |d - d_ref| / max |d_ref|) FP16 1.1e-3 / 1.1e-3, BF16 7.8e-3 / 6.3e-3,
+# BF16-SR 1.3e-2 / 1.2e-2; KE relative error FP16 6.7e-5 / 4.9e-5, BF16
+# 1.2e-3 / 1.1e-4, BF16-SR 9.9e-5 / 1.8e-4.  The bounds carry ~3x headroom.
 SIXTEEN = {"f16": {"dev": 3e-3, "ke": 2e-4, "mass": 1e-6},
-           "bf16": {"dev": 2.5e-2, "ke": 4e-3, "mass": 1e-5}}
+           "bf16": {"dev": 2.5e-2, "ke": 4e-3, "mass": 1e-5},
+           "bf16sr": {"dev": 4e-2, "ke": 6e-4, "mass": 1e-5}}
 
 
-@pytest.mark.parametrize("storage", ["f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f16", "bf16", "bf16sr"])
 @pytest.mark.parametrize("case", ["tg-periodic", "ywalls-moving"])
 def test_sixteen_bit_storage_tracks_fp64_oracle(amd, storage, case):
     from lattice_boltzmann_d3q19_amd.initial import taylor_green
@@ -304,3 +307,26 @@ def test_library_loaded_before_torch_shares_one_hip_runtime():
             "r, u = lat.moments(); assert abs(r.mean() - 1) < 1e-12; print('ok')") % (str(ROOT), str(PKG_DIR))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=110)
     assert r.returncode == 0 and r.stdout.strip().endswith("ok"), r.stderr[-2000:]
+
+
+def test_bf16_stochastic_rounding_is_unbiased(amd):
+    """A deviation 30% of the way between two BF16 values: nearest rounding
+    stores the lower one everywhere; stochastic rounding stores the two
+    neighbours with frequencies 0.7 / 0.3, so the mean is the value itself."""
+    print_case("bf16sr-unbiased")
+    _, w, _, _ = oracle.constants()
+    lo = np.float32(np.uint32(0x3A830000).view(np.float32))   # ~1.0e-3, a BF16 value
+    hi = np.float32(np.uint32(0x3A840000).view(np.float32))   # the next BF16 value
+    d0 = float(lo) + 0.3 * (float(hi) - float(lo))
+    shape = (64, 64, 64)
+    f = np.broadcast_to((w + d0)[:, None, None, None], (19,) + shape).copy()
+    means = {}
+    for storage in ("bf16", "bf16sr"):
+        lat = amd.Lattice(shape, 0.6, storage=storage).set_populations(f)
+        d = lat.populations(np.float32) - w.astype(np.float32)[:, None, None, None]
+        vals = np.unique(d)
+        assert set(vals.tolist()) <= {float(lo), float(hi)}, vals
+        means[storage] = float(d.astype(np.float64).mean())
+    ulp = float(hi) - float(lo)
+    assert abs(means["bf16"] - float(lo)) < 1e-6 * ulp
+    assert abs(means["bf16sr"] - d0) < 0.01 * ulp

@@ -41,7 +41,7 @@ def seeded(shape, seed):
     return f, F
 
 
-@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16", "bf16sr"])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
 @pytest.mark.parametrize("forcing", ["none", "field", "uniform-brinkman"])
 def test_periodic_equals_faceless_halfway_bitwise(amd, storage, shape, forcing):
