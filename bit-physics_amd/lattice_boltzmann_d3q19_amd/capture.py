@@ -177,6 +177,20 @@ def _metadata(manifest: CaptureManifest) -> dict:
     }
 
 
+def _write_npz(path: Path, arrays: dict) -> None:
+    """An .npz archive (a zip of .npy members, readable by numpy.load) whose
+    bytes depend only on the arrays: members are stored uncompressed with a
+    fixed timestamp, so equal captures have equal checksums."""
+    import zipfile
+
+    with zipfile.ZipFile(path, "w", compression=zipfile.ZIP_STORED, allowZip64=True) as zf:
+        for key, arr in arrays.items():
+            info = zipfile.ZipInfo(key + ".npy", date_time=(1980, 1, 1, 0, 0, 0))
+            info.compress_type = zipfile.ZIP_STORED
+            with zf.open(info, "w", force_zip64=True) as fh:
+                np.lib.format.write_array(fh, np.asanyarray(arr), allow_pickle=False)
+
+
 def write_capture(state_iter: Iterable[StepState], manifest: CaptureManifest, out_dir,
                   payload_format: str | None = None) -> Path:
     """Write the payload and the manifest JSON; return the manifest's path.
@@ -223,8 +237,7 @@ def write_capture(state_iter: Iterable[StepState], manifest: CaptureManifest, ou
                 arrays[f"steps/{st.step}/diagnostics/{name}"] = np.asarray(v)
         for k, v in meta.items():
             arrays[f"metadata/{k}"] = np.asarray(v)
-        with payload_path.open("wb") as fh:
-            np.savez(fh, **arrays)
+        _write_npz(payload_path, arrays)
     d = manifest.to_dict()
     d["payload"] = dict(d["payload"], format=fmt, path=payload_path.name,
                         checksum="sha256:" + _sha256(payload_path))

@@ -158,3 +158,17 @@ def test_vector_field_definition():
     v = np.zeros((3, nx, ny, nz))
     v[0] = np.sin(2 * np.pi * y / ny)[None, :, None]
     assert vector_field_diagnostics(v, dx)["check_divergence_free"] == 0.0
+
+
+def test_npz_payload_bytes_are_reproducible(tmp_path):
+    """Equal frames give byte-identical payloads (fixed member timestamps)."""
+    import time
+
+    a = write_capture(frames(), manifest(), tmp_path / "a", payload_format="npz")
+    time.sleep(2.1)  # a zip timestamp has 2 s resolution
+    b = write_capture(frames(), manifest(), tmp_path / "b", payload_format="npz")
+    ma, mb = json.loads(a.read_text()), json.loads(b.read_text())
+    assert ma["payload"]["checksum"] == mb["payload"]["checksum"]
+    assert (tmp_path / "a" / "stub.npz").read_bytes() == (tmp_path / "b" / "stub.npz").read_bytes()
+    c = write_capture(frames(shift=1e-9), manifest(), tmp_path / "c", payload_format="npz")
+    assert json.loads(c.read_text())["payload"]["checksum"] != ma["payload"]["checksum"]

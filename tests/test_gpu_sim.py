@@ -135,3 +135,39 @@ def test_couette_capture_matches_oracle_capture(sim, tmp_path):
     ref = oracle_capture(sim, tmp_path / "cpu", (32, 16, 3), 100, 25, wall=sim.CANONICAL_COUETTE_WALL_VELOCITY)
     d = compare_captures(ours, ref, rtol=1e-10)
     assert d.bit_exact, (d.max_abs_err, d.max_rel_err, d.mismatched_fields[:3])
+
+
+def _locked_manifest():
+    """The diagnostic-tier manifest this engine emits, minus its volatile
+    fields (packages/lattice-boltzmann-d3q19/tests/test_manifest_equality.py:
+    the run's wall clock, its start time and the payload checksum)."""
+    return {
+        "schema_version": "1.0.0",
+        "sim": {"name": "lattice-boltzmann-d3q19", "category": "lattice", "variant": "bgk-d3q19-qian-1992"},
+        "stack": {"name": "mi355x-hip", "version": "0.1.0", "build_id": "libd3q19-gfx950"},
+        "config": {"tier": "diagnostic", "dims": [16, 8, 3], "dtype": "f64", "seed": 42,
+                   "params": {"descriptor": "poiseuille-16x8-seed42-step50", "tau": 0.7, "force_x_lattice": 1.0e-5,
+                              "nz_convention": "depth-3-z-periodic-slab",
+                              "boundary": "bounce-back-y-walls-periodic-xz", "storage": "f64"}},
+        "run": {"step_count": 50, "capture_interval": 10},
+        "payload": {"format": "npz", "path": "poiseuille-16x8-seed42-step50.npz"},
+        "determinism": {"claimed": "bit-exact-same-hw", "atomic_ops": False, "subgroup_ops": False},
+    }
+
+
+def test_diagnostic_manifest_locked_and_stable(sim, tmp_path):
+    import json
+
+    print_case("sim-manifest-lock")
+    got = []
+    for k in range(2):
+        m = json.loads(sim.sim_runner_diagnostic(42, tmp_path / str(k), payload_format="npz").read_text())
+        assert m["run"]["wall_clock_seconds"] >= 0 and m["run"]["start_utc"].endswith("Z")
+        assert m["payload"]["checksum"].startswith("sha256:") and len(m["payload"]["checksum"]) == 71
+        got.append(m)
+        volatile = {"wall_clock_seconds", "start_utc"}
+        locked = {k2: (v if k2 != "run" else {a: b for a, b in v.items() if a not in volatile}) for k2, v in m.items()}
+        locked["payload"] = {a: b for a, b in m["payload"].items() if a != "checksum"}
+        assert locked == _locked_manifest()
+    # run to run the deterministic subset, checksum included, is identical
+    assert got[0]["payload"]["checksum"] == got[1]["payload"]["checksum"]
