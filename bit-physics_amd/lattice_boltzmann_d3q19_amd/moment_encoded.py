@@ -40,14 +40,19 @@ def pad_ranges(lo, hi, margin=RANGE_MARGIN):
 
 
 def calibrate_moment_ranges(shape, tau, steps, *, margin=RANGE_MARGIN, warmup_steps=WARMUP_STEPS,
-                            populations=None, rho=None, u=None, **lattice_kw):
+                            populations=None, rho=None, u=None, prepare=None, **lattice_kw):
     """(lo, hi) of the 19 moments over the FP64 trajectory of ``steps`` steps
     (at least ``warmup_steps``) from the given initial state, padded.
 
     ``lattice_kw`` are the Lattice arguments of the run (boundary, walls,
-    force, ...); the same configuration must be used for the "me16" lattice.
+    force, ...) and ``prepare(lattice)`` sets what they do not (solids, a
+    Brinkman field, a force field); the "me16" lattice must be configured the
+    same way.
     """
-    lat = _load_state(Lattice(shape, tau, storage="f64", **lattice_kw), populations, rho, u)
+    lat = Lattice(shape, tau, storage="f64", **lattice_kw)
+    if prepare is not None:
+        prepare(lat)
+    lat = _load_state(lat, populations, rho, u)
     lo, hi = lat.moment_range()
     for _ in range(max(int(warmup_steps), int(steps))):
         lat.step(1)
@@ -59,12 +64,16 @@ def calibrate_moment_ranges(shape, tau, steps, *, margin=RANGE_MARGIN, warmup_st
 
 
 def moment_encoded_lattice(shape, tau, steps, *, ranges=None, margin=RANGE_MARGIN, warmup_steps=WARMUP_STEPS,
-                           populations=None, rho=None, u=None, **lattice_kw):
+                           populations=None, rho=None, u=None, prepare=None, **lattice_kw):
     """An "me16" lattice with calibrated (or given) ranges and the initial
-    state loaded; ``steps`` is the horizon the ranges must cover."""
+    state loaded; ``steps`` is the horizon the ranges must cover and
+    ``prepare(lattice)`` is applied to the calibration and the "me16"
+    lattice alike."""
     if ranges is None:
         ranges = calibrate_moment_ranges(shape, tau, steps, margin=margin, warmup_steps=warmup_steps,
-                                         populations=populations, rho=rho, u=u, **lattice_kw)
+                                         populations=populations, rho=rho, u=u, prepare=prepare, **lattice_kw)
     lat = Lattice(shape, tau, storage="me16", **lattice_kw)
     lat.set_moment_ranges(*ranges)
+    if prepare is not None:
+        prepare(lat)
     return _load_state(lat, populations, rho, u)
