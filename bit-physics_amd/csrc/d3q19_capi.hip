@@ -546,6 +546,53 @@ int launch_moment_range(d3q19_lattice* L) {
 }
 }  // namespace
 
+namespace {
+template <class Tag, bool ROWS>
+void launch_health_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p,
+                     const typename Codec<Tag>::raw* in) {
+    switch (L->cfg.boundary) {
+        case D3Q19_BC_PERIODIC:
+            k_health<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
+            break;
+        case D3Q19_BC_YWALLS:
+            k_health<Tag, BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
+            break;
+        default:
+            k_health<Tag, BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->partials);
+            break;
+    }
+}
+
+template <class Tag>
+int launch_health(d3q19_lattice* L) {
+    uint32_t nblk;
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        if (int e = me_decode(L)) return e;
+        Params<float> p = make_params<float>(L);
+        p.local = 1;
+        const StepGrid sg = step_grid<TF32>(L);
+        if (L->rows)
+            launch_health_l<TF32, true>(L, sg, p, me_work(L));
+        else
+            launch_health_l<TF32, false>(L, sg, p, me_work(L));
+        nblk = sg.nblocks;
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            launch_health_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+        else
+            launch_health_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
+        nblk = sg.nblocks;
+    }
+    HIPCHK(hipGetLastError());
+    k_health_final<<<NHEALTH, 256, 0, L->stream>>>(L->partials, nblk, L->dout);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* d3q19_last_error(void) { return g_err.c_str(); }
@@ -899,6 +946,54 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
         lo[k] = out[k];
         hi[k] = out[Q + k];
     }
+    return D3Q19_OK;
+}
+
+size_t d3q19_state_bytes(const d3q19_lattice* L) {
+    if (!L) return 0;
+    return L->me ? L->lat_bytes / 2 : L->lat_bytes;
+}
+
+int d3q19_read_state(d3q19_lattice* L, void* dst, int32_t* local, int64_t* steps) {
+    if (int e = check_lat(L)) return e;
+    if (!dst) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    const void* src = L->me ? L->buf[0] : L->buf[L->cur];
+    HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    if (local) *local = L->local ? 1 : 0;
+    if (steps) *steps = L->steps;
+    return D3Q19_OK;
+}
+
+int d3q19_write_state(d3q19_lattice* L, const void* src, int32_t local, int64_t steps) {
+    if (int e = check_lat(L)) return e;
+    if (!src) return fail(D3Q19_EINVAL, "null state");
+    if (steps < 0) return fail(D3Q19_EINVAL, "negative step count");
+    if (L->me && !L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
+    void* dst = L->me ? L->buf[0] : L->buf[L->cur];
+    HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyHostToDevice, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->has_state = true;
+    L->local = L->me ? true : local != 0;
+    L->steps = steps;
+    return D3Q19_OK;
+}
+
+int d3q19_set_step_count(d3q19_lattice* L, int64_t n) {
+    if (int e = check_lat(L)) return e;
+    if (n < 0) return fail(D3Q19_EINVAL, "negative step count");
+    L->steps = n;
+    return D3Q19_OK;
+}
+
+int d3q19_health(d3q19_lattice* L, double out[4]) {
+    if (int e = check_lat(L)) return e;
+    if (!out) return fail(D3Q19_EINVAL, "null output");
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (int rc = D3Q19_DISPATCH(L, launch_health, L)) return rc;
+    HIPCHK(hipMemcpyAsync(out, L->dout, NHEALTH * sizeof(double), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
     return D3Q19_OK;
 }
 

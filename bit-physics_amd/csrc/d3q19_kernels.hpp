@@ -171,6 +171,87 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     }
 }
 
+// Health of f^n (the runners' tier-1 check, NP/sim.py:135-145, on the GPU):
+// per-block [non-finite cells, min rho, max rho, max |u|]; a cell is
+// non-finite when any population, rho or u is NaN or infinite (its rho and u
+// are then left out of the extremes).
+constexpr int NHEALTH = 4;
+
+template <class Tag, int BC, bool ROWS = true>
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_health(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
+             double* __restrict__ partials) {
+    typedef typename Codec<Tag>::C C;
+    constexpr int NW = step_block<Tag>() / 64;
+    __shared__ double red[NW][NHEALTH];
+    double v[NHEALTH] = {0.0, 1e300, -1e300, 0.0};
+    uint32_t x, y, z;
+    const bool live = cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z);
+    const uint32_t cell = live ? (x * p.ny + y) * p.nz + z : 0u;
+    if (live && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
+        C d[Q];
+        state_at<Tag, BC>(in, p, x, y, z, d);
+        bool finite = true;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) finite = finite && isfinite((double)d[i]);
+        C rho, ux, uy, uz;
+        readback_moments(d, p, cell, rho, ux, uy, uz);
+        const double u2 = (double)ux * ux + (double)uy * uy + (double)uz * uz;
+        finite = finite && isfinite((double)rho) && isfinite(u2);
+        if (finite) {
+            v[1] = v[2] = (double)rho;
+            v[3] = sqrt(u2);
+        } else {
+            v[0] = 1.0;
+        }
+    }
+    const int t = threadIdx.x + blockDim.x * threadIdx.y;
+    const int lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        v[0] += __shfl_xor(v[0], o, 64);
+        v[1] = fmin(v[1], __shfl_xor(v[1], o, 64));
+        v[2] = fmax(v[2], __shfl_xor(v[2], o, 64));
+        v[3] = fmax(v[3], __shfl_xor(v[3], o, 64));
+    }
+    if (lane == 0)
+        for (int k = 0; k < NHEALTH; ++k) red[wave][k] = v[k];
+    __syncthreads();
+    if (t == 0) {
+        const int nw = (int)(blockDim.x * blockDim.y) / 64;
+        for (int w = 1; w < nw; ++w) {
+            v[0] += red[w][0];
+            v[1] = fmin(v[1], red[w][1]);
+            v[2] = fmax(v[2], red[w][2]);
+            v[3] = fmax(v[3], red[w][3]);
+        }
+        for (int k = 0; k < NHEALTH; ++k) partials[(size_t)block_linear_id() * NHEALTH + k] = v[k];
+    }
+}
+
+// Final reduction of the health partials: one 256-thread block per slot.
+static __global__ void __launch_bounds__(256)
+    k_health_final(const double* __restrict__ partials, uint32_t nblocks, double* __restrict__ out) {
+    __shared__ double red[4];
+    const int k = blockIdx.x;
+    double v = k == 1 ? 1e300 : (k == 2 ? -1e300 : 0.0);
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+        const double q = partials[(size_t)b * NHEALTH + k];
+        v = k == 0 ? v + q : (k == 1 ? fmin(v, q) : fmax(v, q));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double q = __shfl_xor(v, o, 64);
+        v = k == 0 ? v + q : (k == 1 ? fmin(v, q) : fmax(v, q));
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) v = k == 0 ? v + red[w] : (k == 1 ? fmin(v, red[w]) : fmax(v, red[w]));
+        out[k] = v;
+    }
+}
+
 // Fixed-order final sum of per-block partials (one block).
 static __global__ void __launch_bounds__(BLOCK)
     k_sum_partials(const double* __restrict__ partials, uint32_t nblocks, int ncomp,

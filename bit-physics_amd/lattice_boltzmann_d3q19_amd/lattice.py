@@ -101,6 +101,7 @@ class Lattice:
             for a in range(3):
                 cfg.face_velocity[FACES[face]][a] = float(vel[a])
         self._force_field = None
+        self._me_ranges = None
         if force is None:
             cfg.force_mode = FORCE_NONE
         elif np.ndim(force) == 1:
@@ -281,6 +282,58 @@ class Lattice:
         call("d3q19_diagnostics", self._h, out)
         return {"mass": out[0], "momentum": (out[1], out[2], out[3]), "kinetic_energy": out[4]}
 
+    # ---- checkpoint / restore -----------------------------------------
+    def checkpoint(self, path):
+        """Write the stored state (the population buffer in its storage format
+        and layout, whether it holds f^n in place, the step count) and, for
+        "me16", the moment ranges to ``path`` (.npz).  Configuration and
+        fields (force field, solids, Brinkman) are not saved: restore into a
+        lattice created and set up the same way."""
+        lib = load()
+        raw = np.empty(int(lib.d3q19_state_bytes(self._h)), np.uint8)
+        local = ctypes.c_int32()
+        steps = ctypes.c_int64()
+        self._sync_stream()
+        call("d3q19_read_state", self._h, ctypes.c_void_p(raw.ctypes.data), ctypes.byref(local),
+             ctypes.byref(steps))
+        arrays = {"state": raw, "local": np.int32(local.value), "step_count": np.int64(steps.value),
+                  "shape": np.asarray(self.shape, np.int64), "tau": np.float64(self.tau),
+                  "storage": np.asarray(self.storage), "boundary": np.asarray(self.boundary)}
+        if self._me_ranges is not None:
+            arrays["moment_lo"], arrays["moment_hi"] = self._me_ranges
+        with open(path, "wb") as fh:
+            np.savez(fh, **arrays)
+        return path
+
+    def restore(self, path):
+        """Load a checkpoint written by :meth:`checkpoint`; the continued run
+        repeats the uninterrupted one bit for bit, in every storage."""
+        with np.load(path, allow_pickle=False) as z:
+            if tuple(z["shape"].tolist()) != self.shape or float(z["tau"]) != self.tau:
+                raise ValueError("checkpoint shape / tau differ from this lattice")
+            if str(z["storage"]) != self.storage or str(z["boundary"]) != self.boundary:
+                raise ValueError(f"checkpoint storage/boundary {z['storage']}/{z['boundary']} differ "
+                                 f"from {self.storage}/{self.boundary}")
+            raw = np.ascontiguousarray(z["state"])
+            if raw.nbytes != int(load().d3q19_state_bytes(self._h)):
+                raise ValueError("checkpoint state size differs from this lattice")
+            if "moment_lo" in z.files and self._me_ranges is None:
+                self.set_moment_ranges(z["moment_lo"], z["moment_hi"])
+            self._sync_stream()
+            call("d3q19_write_state", self._h, ctypes.c_void_p(raw.ctypes.data), int(z["local"]),
+                 int(z["step_count"]))
+        return self
+
+    def health(self) -> dict:
+        """Tier-1 health of f^n on the GPU (NP/sim.py:135-145): check_health
+        0.0 when every population, rho and u is finite, else 1.0; the number of
+        non-finite cells; rho range and largest |u| over the finite cells."""
+        self._sync_stream()
+        out = (ctypes.c_double * 4)()
+        call("d3q19_health", self._h, out)
+        return {"check_health": 0.0 if out[0] == 0 else 1.0, "nonfinite_cells": int(out[0]),
+                "rho_min": out[1], "rho_max": out[2], "u_max_lat": out[3]}
+
     # ---- moment-encoded storage ----------------------------------------
     def set_moment_ranges(self, lo, hi):
         """Quantization ranges [lo_k, hi_k] of the 19 moments of f ("me16"
@@ -290,6 +343,7 @@ class Lattice:
         if lo.shape != (19,) or hi.shape != (19,):
             raise ValueError("lo and hi must hold 19 values")
         call("d3q19_set_moment_ranges", self._h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
+        self._me_ranges = (lo.copy(), hi.copy())
         return self
 
     def moment_range(self):

@@ -113,10 +113,27 @@ int d3q19_read_moments(d3q19_lattice* lat, void* rho, void* u, int32_t dtype, in
 /* FP64 reductions over fluid cells of f^n: out[0] mass, out[1..3] momentum
    sum(c f), out[4] kinetic energy 0.5 sum(rho |u|^2) with the read-back u. */
 int d3q19_diagnostics(d3q19_lattice* lat, double out[5]);
+/* Health of f^n in one pass (the runners' tier-1 check, NP/sim.py:135-145):
+ * out = [cells with a NaN/Inf population, rho or u; min rho; max rho;
+ * max |u|] over the finite cells (solid cells excluded). */
+int d3q19_health(d3q19_lattice* lat, double out[4]);
 /* FP64 enstrophy 0.5 sum |curl u|^2 (periodic central differences) of the read-back u. */
 int d3q19_enstrophy(d3q19_lattice* lat, double* out);
 int d3q19_synchronize(d3q19_lattice* lat);
 int64_t d3q19_step_count(const d3q19_lattice* lat);
+/* Set the step counter (the BF16SR rounding bits are keyed on it). */
+int d3q19_set_step_count(d3q19_lattice* lat, int64_t n);
+
+/* Checkpoint / resume of the stored state itself (no reference counterpart:
+ * the reference has no restart path).  The state is the lattice's population
+ * buffer in its storage format and internal layout -- an opaque blob of
+ * d3q19_state_bytes() bytes, valid for a lattice of the same shape, storage
+ * (and, for ME16, moment ranges) -- plus whether it holds f^n in place
+ * (`local`) and the step count.  Writing it back and stepping repeats the
+ * uninterrupted run bit for bit in every storage.  Host memory. */
+size_t d3q19_state_bytes(const d3q19_lattice* lat);
+int d3q19_read_state(d3q19_lattice* lat, void* dst, int32_t* local, int64_t* steps);
+int d3q19_write_state(d3q19_lattice* lat, const void* src, int32_t local, int64_t steps);
 size_t d3q19_device_bytes(const d3q19_lattice* lat);
 
 /* ---- 16-bit moment-encoded storage (D3Q19_ME16) -------------------------
