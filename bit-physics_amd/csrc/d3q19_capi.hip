@@ -384,7 +384,7 @@ int launch_diag(d3q19_lattice* L) {
         nblk = sg.nblocks;
     }
     HIPCHK(hipGetLastError());
-    k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, nblk, NDIAG, L->dout);
+    k_sum_partials<<<NDIAG, BLOCK, 0, L->stream>>>(L->partials, nblk, NDIAG, L->dout);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }

@@ -252,24 +252,25 @@ static __global__ void __launch_bounds__(256)
     }
 }
 
-// Fixed-order final sum of per-block partials (one block).
+// Final sum of per-block partials: one 256-thread block per component, each
+// summing in a fixed order (strided per thread, then a fixed tree), so the
+// result is the same bit for bit on every run.
 static __global__ void __launch_bounds__(BLOCK)
     k_sum_partials(const double* __restrict__ partials, uint32_t nblocks, int ncomp,
                    double* __restrict__ out) {
     __shared__ double red[BLOCK / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    for (int k = 0; k < ncomp; ++k) {
-        double s = 0.0;
-        for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) s += partials[(size_t)b * ncomp + k];
-        s = wave_sum(s);
-        if (lane == 0) red[wave] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double t = 0.0;
-            for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
-            out[k] = t;
-        }
-        __syncthreads();
+    const int k = blockIdx.x;
+    if (k >= ncomp) return;
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) s += partials[(size_t)b * ncomp + k];
+    s = wave_sum(s);
+    if (lane == 0) red[wave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < BLOCK / 64; ++w) t += red[w];
+        out[k] = t;
     }
 }
 

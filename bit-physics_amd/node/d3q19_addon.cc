@@ -12,6 +12,7 @@
 //   lat.diagnostics();                   // {mass, momentum: [3], kineticEnergy}
 //   lat.setMomentRanges(lo, hi);         // "me16" storage: moment quantization ranges
 //   lat.momentRange();                   // {lo, hi} of the 19 moments of the state
+//   lat.health();                        // {checkHealth, nonfiniteCells, rhoMin, rhoMax, uMaxLat}
 //   lat.close();
 //
 // All arrays are host memory; the library stages them to the GPU.
@@ -337,6 +338,26 @@ napi_value MomentRange(napi_env env, napi_callback_info info) {
     return obj;
 }
 
+// health(): {checkHealth, nonfiniteCells, rhoMin, rhoMax, uMaxLat} of the state.
+napi_value Health(napi_env env, napi_callback_info info) {
+    size_t argc = 0;
+    Wrapped* w = unwrap(env, info, &argc, nullptr);
+    if (!w) return nullptr;
+    double h[4];
+    int st = d3q19_health(w->lat, h);
+    if (st) return throw_status(env, "d3q19_health", st);
+    const char* names[5] = {"checkHealth", "nonfiniteCells", "rhoMin", "rhoMax", "uMaxLat"};
+    const double vals[5] = {h[0] == 0.0 ? 0.0 : 1.0, h[0], h[1], h[2], h[3]};
+    napi_value obj;
+    NAPI_CALL(env, napi_create_object(env, &obj));
+    for (int k = 0; k < 5; ++k) {
+        napi_value v;
+        NAPI_CALL(env, napi_create_double(env, vals[k], &v));
+        NAPI_CALL(env, napi_set_named_property(env, obj, names[k], v));
+    }
+    return obj;
+}
+
 napi_value StepCount(napi_env env, napi_callback_info info) {
     size_t argc = 0;
     Wrapped* w = unwrap(env, info, &argc, nullptr);
@@ -376,6 +397,7 @@ napi_value Init(napi_env env, napi_value exports) {
         {"stepCount", nullptr, StepCount, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"setMomentRanges", nullptr, SetMomentRanges, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"momentRange", nullptr, MomentRange, nullptr, nullptr, nullptr, napi_default, nullptr},
+        {"health", nullptr, Health, nullptr, nullptr, nullptr, napi_default, nullptr},
         {"close", nullptr, Close, nullptr, nullptr, nullptr, napi_default, nullptr},
     };
     napi_value cls;

@@ -33,6 +33,14 @@ export interface Diagnostics {
   kineticEnergy: number;
 }
 
+export interface Health {
+  checkHealth: number;          // 0 when every population, rho and u is finite, else 1
+  nonfiniteCells: number;
+  rhoMin: number;
+  rhoMax: number;
+  uMaxLat: number;              // max |u| over the finite cells
+}
+
 export declare class Lattice {
   constructor(cfg: LatticeConfig);
   readonly shape: Vec3;
@@ -47,6 +55,7 @@ export declare class Lattice {
   setMomentRanges(lo: Float64Array, hi: Float64Array): void;
   // minimum and maximum over all cells of the 19 moments of the current state
   momentRange(): { lo: Float64Array; hi: Float64Array };
+  health(): Health;
   close(): void;
 }
 

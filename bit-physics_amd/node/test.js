@@ -23,6 +23,9 @@ const out = { version: version() };
   lat.step(1000);
   const { rho, u } = lat.readMoments();
   out.poiseuille_umax = maxSpeed(u, n).toExponential(7);
+  const h = lat.health();
+  assert.strictEqual(h.checkHealth, 0);
+  assert.strictEqual(h.uMaxLat.toExponential(7), out.poiseuille_umax);
   out.poiseuille_rhomax = Math.max(...rho).toExponential(7);
   assert.strictEqual(out.poiseuille_umax, '8.6508320e-3');
   assert.strictEqual(out.poiseuille_rhomax, '1.0000000e+0');
