@@ -108,3 +108,35 @@ def test_node_addon_loads_without_gpu():
                         str(ROOT / "bit-physics_amd" / "node" / "index.js")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("function d3q19")
+
+
+# reference/__init__.py:43-71 of packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19
+REFERENCE_SURFACE = {
+    "C", "CANONICAL_COUETTE_NX", "CANONICAL_COUETTE_NY", "CANONICAL_COUETTE_STEPS",
+    "CANONICAL_DESCRIPTOR_COUETTE", "CANONICAL_DESCRIPTOR_POISEUILLE", "CANONICAL_NZ",
+    "CANONICAL_POISEUILLE_NX", "CANONICAL_POISEUILLE_NY", "CANONICAL_POISEUILLE_STEPS",
+    "CANONICAL_SEED", "CS2", "VELOCITIES", "W", "WEIGHTS", "apply_bounce_back_y_walls", "bgk",
+    "bgk_step", "density_field", "density_moment", "equilibrium", "feq", "feq_field",
+    "macroscopic_velocity", "momentum_field", "momentum_moment", "stream",
+}
+
+
+def test_dropin_surface_matches_reference():
+    """Same public names and submodule import paths as the reference package
+    (its tests import ``reference.bgk.bgk_step`` and ``reference.equilibrium``)."""
+    import importlib
+
+    import numpy as np
+
+    R = importlib.import_module("lattice_boltzmann_d3q19_amd.reference")
+    assert set(R.__all__) == REFERENCE_SURFACE
+    assert all(hasattr(R, n) for n in REFERENCE_SURFACE)
+    bgk = importlib.import_module("lattice_boltzmann_d3q19_amd.reference.bgk")
+    eq = importlib.import_module("lattice_boltzmann_d3q19_amd.reference.equilibrium")
+    assert R.bgk is bgk and R.equilibrium is eq and bgk.bgk_step is R.bgk_step
+    assert {"feq_field", "feq", "density_field", "momentum_field"} <= set(dir(eq))
+    # constants.py: weights sum to one, c_i opposite pairs, isotropy sum w c c = cs2 I
+    assert abs(sum(R.WEIGHTS) - 1.0) < 1e-15 and R.CS2 == 1.0 / 3.0
+    assert np.array_equal(R.C[R.OPP], -R.C)
+    assert np.allclose(np.einsum("i,ia,ib->ab", R.W, R.C, R.C), np.eye(3) / 3, rtol=0, atol=1e-15)
+    assert R.CANONICAL_DESCRIPTOR_POISEUILLE == "poiseuille-64x32-seed42-step1000"
