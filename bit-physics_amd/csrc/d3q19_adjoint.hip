@@ -2,14 +2,21 @@
 //
 // Forward map (per-cell force F, optional Brinkman solid fraction s):
 //   f^0 = feq(1, u0),  f^{n+1} = S C(f^n; F - s m),  u^T = (m + (F - s m)/2) / rho  at f^T
-// with C the BGK+Guo collision and S the streaming permutation (periodic, or
-// half-way bounce-back on stationary wall faces -- also a permutation).  The
-// engine stores post-collision states, so the forward runs on
+// with C the BGK+Guo collision and S the streaming with its boundary:
+// periodic, half-way bounce-back on wall faces (a permutation plus the
+// constant moving-wall term 6 w_i c_i.u_face), or the reference y-walls
+// S = B P (periodic streaming P, then the wall map B on rows 0 and ny - 1,
+// affine in the streamed values: reference/bgk.py:190-215).  The engine
+// stores post-collision states, so the forward runs on
 //   H_0 = S^-1 f^0,  H_{j+1} = k_step(H_j),  f^j = S H_j.
+// The y-wall map discards the populations it replaces, so S has no inverse
+// there: H_0 is f^0 itself (local layout), H_1 = C(f^0) by a local collision,
+// and f^0 is read back from H_0 directly.
 // Backward (lambda^n = dL/df^n):
 //   lambda^T   = J_u(f^T)^T dL/du^T
 //   lambda^n   = J_C(f^n)^T S^T lambda^{n+1},
-//                (S^T mu)_i(x) = mu_i(x + c_i), or mu_opp(i)(x) when x + c_i is across a wall
+//                (S^T mu)_i(x) = mu_i(x + c_i), or mu_opp(i)(x) when x + c_i is across a wall;
+//                y-walls: S^T = P^T B^T, B^T applied on the wall rows first
 //   dL/dF, dL/ds accumulate the parameter parts of each step's Jacobian
 //   dL/du0     = dfeq(1,u0)/du0^T lambda^0
 // States between checkpoints (every K steps) are recomputed by the forward
@@ -49,7 +56,11 @@ __device__ __forceinline__ C feq_dev(int i, const C* u0, uint32_t N, uint32_t c)
 }
 
 // Initial post-collision pre-image H_0 = S^-1 feq(1, u0):
-//   H_0,i(x) = feq_i(1, u0(x + c_i)) - w_i, or feq_opp(i)(1, u0(x)) - w when x + c_i is across a wall.
+//   H_0,i(x) = feq_i(1, u0(x + c_i)) - w_i, or, when x + c_i is across a wall,
+//   feq_opp(i)(1, u0(x)) - w_i + 6 w_i (c_i . u_face), u_face the velocity of the
+//   face that the pull of direction opp(i) at x reflects from (the pull adds
+//   6 w_opp (c_opp . u_face), which cancels it).
+// Reference y-walls: H_0 = f^0 - w in local layout (S has no inverse).
 template <class Tag, int BC>
 __global__ void __launch_bounds__(step_block<Tag>())
     k_adj_unstream(const typename Codec<Tag>::C* __restrict__ u0, typename Codec<Tag>::raw* __restrict__ out,
@@ -62,8 +73,12 @@ __global__ void __launch_bounds__(step_block<Tag>())
     C d[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        if (BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p)) {
-            d[i] = feq_dev<C>(OPP[i], u0, N, self);
+        const int face = BC == BC_HALFWAY ? halfway_face(OPP[i], p.wall_mask, x, y, z, p) : -1;
+        if (BC == BC_YWALLS) {
+            d[i] = feq_dev<C>(i, u0, N, self);
+        } else if (face >= 0) {
+            const C cu = cdot<C>(i, p.face_u[face][0], p.face_u[face][1], p.face_u[face][2]);
+            d[i] = feq_dev<C>(OPP[i], u0, N, self) + C(6) * wgt<C>(i) * cu;
         } else {
             const uint32_t sx = (x + p.nx + CX[i]) % p.nx, sy = (y + p.ny + CY[i]) % p.ny,
                            sz = (z + p.nz + CZ[i]) % p.nz;
@@ -109,7 +124,9 @@ __global__ void __launch_bounds__(step_block<Tag>())
 
 // One reverse step: lambda_out = J_C(f^n)^T S^T lambda_in; the parameter
 // parts go to gF (force) and gs (Brinkman fraction).
-template <class Tag, int BC, bool BRINK>
+// LOCAL0: f^n is stored in local layout (the y-wall chain's f^0) instead of
+// being pulled from a post-collision state.
+template <class Tag, int BC, bool BRINK, bool LOCAL0 = false>
 __global__ void __launch_bounds__(step_block<Tag>())
     k_adj_step(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
                const typename Codec<Tag>::raw* __restrict__ lam_in, typename Codec<Tag>::raw* __restrict__ lam_out,
@@ -134,7 +151,10 @@ __global__ void __launch_bounds__(step_block<Tag>())
             mu[i] = ld<Tag, true>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
     }
     C d[Q];
-    pull<Tag, BC, true>(H, p, x, y, z, d);
+    if constexpr (LOCAL0)
+        load_local<Tag>(H, p, x, y, z, d);
+    else
+        pull<Tag, BC, true>(H, p, x, y, z, d);
     C drho, mx, my, mz;
     moments(d, drho, mx, my, mz);
     const C rho = C(1) + drho;
@@ -180,6 +200,47 @@ __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
     for (int a = 0; a < 3; ++a) gF[(size_t)a * N + cell] += gt[a];
     if (BRINK) gs[cell] -= gt[0] * m[0] + gt[1] * m[1] + gt[2] * m[2];
+}
+
+// Reference y-walls, S = B P: on row 0 (c_y > 0) and row ny - 1 (c_y < 0) the
+// streamed values d of a cell become
+//   (B d)_i = d_opp(i) - a_i (1 + sum_j d_j)   for the replaced directions i,
+//   (B d)_i = d_i                              for the others,
+// a_i = 6 w_i (c_i . u_wall), the density taken over all 19 streamed values
+// (ywall_swap).  S^T = P^T B^T; this applies B^T in place to an adjoint state
+// on the wall rows:
+//   (B^T l)_j = [j kept] l_j + [opp(j) replaced] l_opp(j) - sum_{i replaced} a_i l_i.
+// One thread per wall cell: nx * rows * nz threads, rows = 2 (1 when ny = 1,
+// where both walls act on the one row).
+template <class Tag>
+__global__ void __launch_bounds__(256)
+    k_adj_ywalls_T(typename Codec<Tag>::raw* __restrict__ lam, Params<typename Codec<Tag>::C> p) {
+    typedef typename Codec<Tag>::C C;
+    const uint32_t rows = p.ny > 1 ? 2u : 1u;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)p.nx * rows * p.nz) return;
+    const uint32_t z = (uint32_t)(t % p.nz);
+    const uint32_t r = (uint32_t)((t / p.nz) % rows);
+    const uint32_t x = (uint32_t)(t / ((uint64_t)p.nz * rows));
+    const uint32_t y = r == 0 ? 0u : p.ny - 1;
+    const bool bot = (y == 0), top = (y + 1 == p.ny);
+    C l[Q];
+    load_local<Tag>(lam, p, x, y, z, l);
+    bool rep[Q];
+    C acc = C(0);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        rep[i] = (bot && CY[i] > 0) || (top && CY[i] < 0);
+        if (rep[i]) {
+            const C cu = CY[i] > 0 ? cdot<C>(i, p.wbot[0], p.wbot[1], p.wbot[2])
+                                   : cdot<C>(i, p.wtop[0], p.wtop[1], p.wtop[2]);
+            acc += C(6) * wgt<C>(i) * cu * l[i];
+        }
+    }
+    C o[Q];
+#pragma unroll
+    for (int j = 0; j < Q; ++j) o[j] = (rep[j] ? C(0) : l[j]) + (rep[OPP[j]] ? l[OPP[j]] : C(0)) - acc;
+    store_local<Tag>(lam, p, x, y, z, o);
 }
 
 // dL/du0_a = sum_i lambda^0_i w_i (3 c_a + 9 cu c_a - 3 u_a)   (rho = 1)
@@ -269,6 +330,11 @@ Params<C> adj_params(const d3q19_adjoint* A, size_t es) {
     p.force = (const C*)A->force;
     p.brink = A->brink ? (const C*)A->s : nullptr;
     p.wall_mask = A->cfg.boundary == D3Q19_BC_HALFWAY ? A->cfg.wall_faces : 0u;
+    for (int a = 0; a < 3; ++a) {
+        p.wtop[a] = (C)A->cfg.wall_velocity_top[a];
+        p.wbot[a] = (C)A->cfg.wall_velocity_bottom[a];
+        for (int k = 0; k < 6; ++k) p.face_u[k][a] = (C)A->cfg.face_velocity[k][a];
+    }
     return p;
 }
 
@@ -282,17 +348,29 @@ void adj_grid(const d3q19_adjoint* A, dim3& grid, dim3& block, uint32_t threads 
     grid = dim3((nz + bz - 1) / bz, (A->cfg.ny + by - 1) / by, A->cfg.nx);
 }
 
+// One forward step H_{j+1} = C(S H_j).  first: j = 0 of a y-wall chain,
+// whose H_0 is f^0 itself, so the step is the local collision H_1 = C(f^0).
 template <class Tag>
-int step_into(d3q19_adjoint* A, void* in, void* out) {
+int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
     const Params<C> p = adj_params<C>(A, A->es);
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
-    if (A->cfg.boundary == D3Q19_BC_HALFWAY)
-        k_step<Tag, BC_HALFWAY, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
-    else
-        k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>((const R*)in, (R*)out, p, nullptr, nullptr);
+    const R* src = (const R*)in;
+    switch (A->cfg.boundary) {
+        case D3Q19_BC_HALFWAY:
+            k_step<Tag, BC_HALFWAY, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            break;
+        case D3Q19_BC_YWALLS:
+            if (first)
+                k_collide_local<Tag, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            else
+                k_step<Tag, BC_YWALLS, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            break;
+        default:
+            k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+    }
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -315,18 +393,15 @@ int convert(d3q19_adjoint* A, const void* in, void* out, bool to_ckpt) {
     return D3Q19_OK;
 }
 
-template <class Tag>
-int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
+template <class Tag, int BC>
+int run_forward_t(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
     const Params<C> p = adj_params<C>(A, A->es);
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
     void* h0 = A->ckpt_same ? A->ckpt[0] : A->seg[0];
-    if (A->cfg.boundary == D3Q19_BC_HALFWAY)
-        k_adj_unstream<Tag, BC_HALFWAY><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)h0, p);
-    else
-        k_adj_unstream<Tag, BC_PERIODIC><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)h0, p);
+    k_adj_unstream<Tag, BC><<<g, b, 0, A->stream>>>((const C*)A->u0, (R*)h0, p);
     D3Q19_HIPCHK(hipGetLastError());
     if (!A->ckpt_same)
         if (int e = convert<Tag>(A, h0, A->ckpt[0], true)) return e;
@@ -339,29 +414,27 @@ int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
             out = A->ckpt[nxt / A->K];
         else
             out = (cur == A->seg[0]) ? A->seg[1] : A->seg[0];
-        if (int e = step_into<Tag>(A, cur, out)) return e;
+        if (int e = step_into<Tag>(A, cur, out, j == 0)) return e;
         if (is_ck && !A->ckpt_same)
             if (int e = convert<Tag>(A, out, A->ckpt[nxt / A->K], true)) return e;
         cur = out;
     }
     A->Hfinal = cur;
-    if (A->cfg.boundary == D3Q19_BC_HALFWAY) {
-        if (out_dt == D3Q19_F64)
-            k_readback<Tag, BC_HALFWAY, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                        (double*)u_out_dev);
-        else
-            k_readback<Tag, BC_HALFWAY, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                       (float*)u_out_dev);
-    } else {
-        if (out_dt == D3Q19_F64)
-            k_readback<Tag, BC_PERIODIC, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                         (double*)u_out_dev);
-        else
-            k_readback<Tag, BC_PERIODIC, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr,
-                                                                        (float*)u_out_dev);
-    }
+    if (out_dt == D3Q19_F64)
+        k_readback<Tag, BC, double><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr, (double*)u_out_dev);
+    else
+        k_readback<Tag, BC, float><<<g, b, 0, A->stream>>>((const R*)cur, p, nullptr, nullptr, (float*)u_out_dev);
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
+}
+
+template <class Tag>
+int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
+    switch (A->cfg.boundary) {
+        case D3Q19_BC_HALFWAY: return run_forward_t<Tag, BC_HALFWAY>(A, u_out_dev, out_dt);
+        case D3Q19_BC_YWALLS: return run_forward_t<Tag, BC_YWALLS>(A, u_out_dev, out_dt);
+        default: return run_forward_t<Tag, BC_PERIODIC>(A, u_out_dev, out_dt);
+    }
 }
 
 template <class Tag, int BC, bool BRINK>
@@ -385,10 +458,19 @@ int run_backward_t(d3q19_adjoint* A) {
         if (!A->ckpt_same)
             if (int e = convert<Tag>(A, A->ckpt[k], A->seg[0], false)) return e;
         for (int32_t j = 1; j < len; ++j)
-            if (int e = step_into<Tag>(A, slot(j - 1), slot(j))) return e;
+            if (int e = step_into<Tag>(A, slot(j - 1), slot(j), k == 0 && j == 1)) return e;
         for (int32_t j = len - 1; j >= 0; --j) {
-            k_adj_step<Tag, BC, BRINK><<<g, b, 0, A->stream>>>((const R*)slot(j), p, (const R*)A->lam[cur],
-                                                                (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
+            if constexpr (BC == BC_YWALLS) {
+                const uint32_t n = p.nx * (p.ny > 1 ? 2u : 1u) * p.nz;
+                k_adj_ywalls_T<Tag><<<(n + 255) / 256, 256, 0, A->stream>>>((R*)A->lam[cur], p);
+                D3Q19_HIPCHK(hipGetLastError());
+            }
+            if (BC == BC_YWALLS && k == 0 && j == 0)
+                k_adj_step<Tag, BC, BRINK, true><<<g, b, 0, A->stream>>>(
+                    (const R*)slot(j), p, (const R*)A->lam[cur], (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
+            else
+                k_adj_step<Tag, BC, BRINK, false><<<g, b, 0, A->stream>>>(
+                    (const R*)slot(j), p, (const R*)A->lam[cur], (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
             D3Q19_HIPCHK(hipGetLastError());
             cur ^= 1;
         }
@@ -400,9 +482,14 @@ int run_backward_t(d3q19_adjoint* A) {
 
 template <class Tag>
 int run_backward(d3q19_adjoint* A) {
-    const bool hw = A->cfg.boundary == D3Q19_BC_HALFWAY;
-    if (hw) return A->brink ? run_backward_t<Tag, BC_HALFWAY, true>(A) : run_backward_t<Tag, BC_HALFWAY, false>(A);
-    return A->brink ? run_backward_t<Tag, BC_PERIODIC, true>(A) : run_backward_t<Tag, BC_PERIODIC, false>(A);
+    switch (A->cfg.boundary) {
+        case D3Q19_BC_HALFWAY:
+            return A->brink ? run_backward_t<Tag, BC_HALFWAY, true>(A) : run_backward_t<Tag, BC_HALFWAY, false>(A);
+        case D3Q19_BC_YWALLS:
+            return A->brink ? run_backward_t<Tag, BC_YWALLS, true>(A) : run_backward_t<Tag, BC_YWALLS, false>(A);
+        default:
+            return A->brink ? run_backward_t<Tag, BC_PERIODIC, true>(A) : run_backward_t<Tag, BC_PERIODIC, false>(A);
+    }
 }
 
 void free_all(d3q19_adjoint* A) {
@@ -444,15 +531,11 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     if (!cfg || !opt || !out) return set_error(D3Q19_EINVAL, "null argument");
     *out = nullptr;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return set_error(D3Q19_EINVAL, "extents must be >= 1");
-    if (cfg->boundary != D3Q19_BC_PERIODIC && cfg->boundary != D3Q19_BC_HALFWAY)
-        return set_error(D3Q19_EINVAL, "adjoint needs a periodic or half-way lattice");
-    if (cfg->boundary == D3Q19_BC_HALFWAY) {
-        if (cfg->wall_faces & ~63u) return set_error(D3Q19_EINVAL, "wall_faces has bits beyond the six faces");
-        for (int k = 0; k < 6; ++k)
-            for (int a = 0; a < 3; ++a)
-                if (cfg->face_velocity[k][a] != 0.0)
-                    return set_error(D3Q19_EINVAL, "adjoint supports stationary walls only");
-    }
+    if (cfg->boundary != D3Q19_BC_PERIODIC && cfg->boundary != D3Q19_BC_HALFWAY &&
+        cfg->boundary != D3Q19_BC_YWALLS)
+        return set_error(D3Q19_EINVAL, "unknown boundary");
+    if (cfg->boundary == D3Q19_BC_HALFWAY && (cfg->wall_faces & ~63u))
+        return set_error(D3Q19_EINVAL, "wall_faces has bits beyond the six faces");
     if (cfg->force_mode != D3Q19_FORCE_FIELD) return set_error(D3Q19_EINVAL, "adjoint needs force_mode FIELD");
     if (cfg->storage != D3Q19_F32 && cfg->storage != D3Q19_F64)
         return set_error(D3Q19_EINVAL, "adjoint storage must be F32 or F64");

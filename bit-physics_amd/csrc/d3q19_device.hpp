@@ -344,6 +344,23 @@ __device__ __forceinline__ void ywall_swap(const Params<C>& p, uint32_t y, C d[Q
     }
 }
 
+// Half-way mode: the wall face (D3Q19_FACE_* index) that the pull source
+// x - c_i of a cell lies across, or -1.  An edge or corner direction at an
+// edge or corner of the box crosses two or three wall faces; the last one in
+// face order counts (its velocity enters the moving-wall term).
+template <class C>
+__device__ __forceinline__ int halfway_face(int i, uint32_t wm, uint32_t x, uint32_t y, uint32_t z,
+                                            const Params<C>& p) {
+    int face = -1;
+    if (CX[i] > 0 && x == 0 && (wm & 1u)) face = 0;
+    if (CX[i] < 0 && x + 1 == p.nx && (wm & 2u)) face = 1;
+    if (CY[i] > 0 && y == 0 && (wm & 4u)) face = 2;
+    if (CY[i] < 0 && y + 1 == p.ny && (wm & 8u)) face = 3;
+    if (CZ[i] > 0 && z == 0 && (wm & 16u)) face = 4;
+    if (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u)) face = 5;
+    return face;
+}
+
 // ---------------------------------------------------------------------------
 // Pull streaming with boundary handling.  Produces the pre-collision
 // deviations d_i(x) = f_i(x) - w_i of the reference state after streaming and
@@ -374,13 +391,7 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
         const char* self = row_of(in, p, x, y);
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            int face = -1;
-            if (CX[i] > 0 && x == 0 && (wm & 1u)) face = 0;
-            if (CX[i] < 0 && x + 1 == p.nx && (wm & 2u)) face = 1;
-            if (CY[i] > 0 && y == 0 && (wm & 4u)) face = 2;
-            if (CY[i] < 0 && y + 1 == p.ny && (wm & 8u)) face = 3;
-            if (CZ[i] > 0 && z == 0 && (wm & 16u)) face = 4;
-            if (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u)) face = 5;
+            const int face = halfway_face(i, wm, x, y, z, p);
             if (face >= 0) {
                 const C cu = cdot<C>(i, p.face_u[face][0], p.face_u[face][1], p.face_u[face][2]);
                 d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;

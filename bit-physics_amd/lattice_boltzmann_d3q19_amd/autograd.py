@@ -8,8 +8,10 @@
 Forward: f^0 = feq(1, u0); ``steps`` BGK+Guo steps with the per-cell force F
 (the reference's ``bgk_step(f, tau, force_lattice=F)``,
 packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/bgk.py:45-112),
-optionally with half-way bounce-back walls and a Brinkman solid fraction s
-(body force F - s m); u^T = ``macroscopic_velocity(f^T, F - s m)``
+optionally followed each step by the reference's
+``apply_bounce_back_y_walls(f, top, bottom)`` (bgk.py:145-216), or with
+half-way bounce-back walls (stationary or moving), and optionally a Brinkman
+solid fraction s (body force F - s m); u^T = ``macroscopic_velocity(f^T, F - s m)``
 (bgk.py:129-142).  Backward: the hand-written adjoint kernel with
 checkpoint-every-K recomputation.  Tensors are passed by pointer on the
 current HIP stream.
@@ -21,8 +23,7 @@ import ctypes
 
 from . import _device as dev
 from ._lib import (
-    BC_HALFWAY,
-    BC_PERIODIC,
+    BOUNDARY,
     D3Q19_F16,
     D3Q19_F32,
     D3Q19_F64,
@@ -44,7 +45,9 @@ class DifferentiableLBM:
     shape, tau, steps : lattice, relaxation time, number of steps T.
     checkpoint_every : K; <= 0 picks the K that needs the least memory.
     storage : "f32" or "f64" -- working precision of forward and adjoint.
-    boundary : "periodic" or "halfway" (stationary walls on ``wall_faces``).
+    boundary : "periodic", "ywalls" (the reference's y-wall bounce-back with
+        ``wall_velocity_top`` / ``wall_velocity_bottom``) or "halfway" (walls on
+        ``wall_faces``, moving with ``face_velocity`` {face: (ux, uy, uz)}).
     brinkman : if True the forward map takes a Brinkman solid fraction s.
     checkpoint_storage : None (= storage) or "f16" (f - w checkpoints).
     memory_budget : bytes the adjoint may hold (0 = unlimited); creation
@@ -55,7 +58,8 @@ class DifferentiableLBM:
     """
 
     def __init__(self, shape, tau, steps, checkpoint_every=0, storage="f32", boundary="periodic",
-                 wall_faces=(), brinkman=False, checkpoint_storage=None, memory_budget=0, device=0):
+                 wall_faces=(), brinkman=False, checkpoint_storage=None, memory_budget=0, device=0,
+                 wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0), face_velocity=None):
         lib = load()
         self.shape = tuple(int(s) for s in shape)
         self.steps = int(steps)
@@ -68,11 +72,20 @@ class DifferentiableLBM:
         cfg.tau = float(tau)
         cfg.storage = STORAGE[storage]
         cfg.force_mode = FORCE_FIELD
-        cfg.boundary = {"periodic": BC_PERIODIC, "halfway": BC_HALFWAY}[boundary]
+        try:
+            cfg.boundary = BOUNDARY[boundary]
+        except KeyError:
+            raise ValueError(f"unknown boundary {boundary!r}") from None
         mask = 0
         for face in wall_faces:
             mask |= 1 << FACES[face]
         cfg.wall_faces = mask
+        for a in range(3):
+            cfg.wall_velocity_top[a] = float(wall_velocity_top[a])
+            cfg.wall_velocity_bottom[a] = float(wall_velocity_bottom[a])
+        for face, vel in (face_velocity or {}).items():
+            for a in range(3):
+                cfg.face_velocity[FACES[face]][a] = float(vel[a])
         cfg.device = self.device
         opt = AdjointOptions()
         lib.d3q19_adjoint_options_default(ctypes.byref(opt))

@@ -188,10 +188,12 @@ int d3q19_macroscopic_velocity(const void* f, const void* force, void* u, int64_
  * The backward pass propagates dL/du^T to dL/du0, dL/dF and dL/ds with a
  * hand-written adjoint kernel (reverse streaming with the bounce-back links
  * reversed + transposed BGK/Guo/Brinkman Jacobian), recomputing the states
- * between checkpoints.  cfg: force_mode D3Q19_FORCE_FIELD, boundary PERIODIC
- * or HALFWAY with stationary walls and no solid cells, storage F32 or F64
- * (the adjoint runs in the storage precision).  Fields are (3,Nx,Ny,Nz) or
- * (Nx,Ny,Nz) device or host arrays of dtype F32/F64. */
+ * between checkpoints.  cfg: force_mode D3Q19_FORCE_FIELD; boundary PERIODIC,
+ * YWALLS (the reference loop bgk_step + apply_bounce_back_y_walls with the
+ * wall velocities, NP/sim.py:312-326) or HALFWAY (stationary or moving
+ * faces, no solid cells); storage F32 or F64 (the adjoint runs in the
+ * storage precision).  Fields are (3,Nx,Ny,Nz) or (Nx,Ny,Nz) device or host
+ * arrays of dtype F32/F64. */
 typedef struct d3q19_adjoint d3q19_adjoint;
 
 typedef struct d3q19_adjoint_options {

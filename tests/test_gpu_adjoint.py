@@ -198,3 +198,109 @@ def test_memory_budget_autotune(mods):
     s.close()
     with pytest.raises(D3Q19Error, match="ENOMEM"):
         ag.DifferentiableLBM(shape, 0.6, 400, checkpoint_every=0, memory_budget=1 << 20)
+
+
+# Moving walls: the reference y-walls (both plates moving) and a half-way box
+# with two moving faces.  The moving-wall terms are constants of the half-way
+# map but enter the y-wall map through the wall density, so its transpose
+# carries a rank-one term.
+YWALLS = dict(boundary="ywalls", wall_velocity_top=(0.05, 0.0, 0.01), wall_velocity_bottom=(-0.02, 0.0, 0.0))
+LID_FACES = {"+y": (0.05, 0.0, 0.02), "-x": (0.0, 0.01, 0.0)}
+LID = dict(boundary="halfway", wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"), face_velocity=LID_FACES)
+
+
+def wall_problem(shape, seed):
+    rng = np.random.default_rng(seed)
+    u0 = 0.01 * rng.standard_normal((3,) + shape)
+    F = 1e-5 * rng.standard_normal((3,) + shape)
+    F[0] += 1e-4
+    s = np.clip(rng.random(shape) * 1.4 - 0.6, 0.0, 1.0)
+    ut = 0.01 * rng.standard_normal((3,) + shape)
+    return u0, F, s, ut
+
+
+@pytest.mark.parametrize("ny", [9, 2, 1])
+def test_ywalls_adjoint_forward_is_the_reference_loop(mods, ny):
+    """u^T of the y-wall chain = macroscopic_velocity(f^T, F) of the reference
+    loop f <- apply_bounce_back_y_walls(bgk_step(f, tau, F)) from feq(1, u0)."""
+    torch, ag = mods
+    print_case("adjoint-ywalls-forward", ny=ny)
+    shape = (6, ny, 5)
+    T, tau = 13, 0.7
+    u0, F, _, _ = wall_problem(shape, 3)
+    solver = ag.DifferentiableLBM(shape, tau, T, checkpoint_every=4, storage="f64", **YWALLS)
+    u = solver.forward_raw(torch.tensor(u0, device="cuda"), torch.tensor(F, device="cuda")).cpu().numpy()
+    f = oracle.run_reference(oracle.feq_field(np.ones(shape), u0), tau, T, force_lattice=F, ywalls=True,
+                             wall_velocity_top=YWALLS["wall_velocity_top"],
+                             wall_velocity_bottom=YWALLS["wall_velocity_bottom"])
+    assert field_scaled_err(u, oracle.macroscopic_velocity(f, F)) < 1e-11
+
+
+def test_moving_lid_adjoint_forward_is_the_halfway_loop(mods):
+    """u^T of the half-way chain with moving faces and Brinkman drag = the
+    oracle's half-way trajectory from feq(1, u0)."""
+    torch, ag = mods
+    print_case("adjoint-lid-forward")
+    shape = (7, 6, 5)
+    T, tau = 11, 0.7
+    u0, F, s, _ = wall_problem(shape, 4)
+    solver = ag.DifferentiableLBM(shape, tau, T, checkpoint_every=3, storage="f64", brinkman=True, **LID)
+    u = solver.forward_raw(*(torch.tensor(v, device="cuda") for v in (u0, F, s))).cpu().numpy()
+    fu = np.zeros((6, 3))
+    fu[3], fu[0] = LID_FACES["+y"], LID_FACES["-x"]
+    f = oracle.run_halfway(oracle.feq_field(np.ones(shape), u0), tau, T, wall_mask=63, face_velocity=fu,
+                           force_field=F, brinkman=s)
+    Feff = F - s[None] * oracle.momentum_field(f)
+    assert field_scaled_err(u, oracle.macroscopic_velocity(f, Feff)) < 1e-11
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-6), ("f32", 2e-4)])
+@pytest.mark.parametrize("case", ["ywalls", "ywalls-ny1", "lid"])
+def test_moving_wall_adjoint_matches_central_differences(mods, case, storage, tol):
+    """<grad, d> in (u0, F, s) vs FP64 central differences of the same forward map."""
+    torch, ag = mods
+    print_case("adjoint-moving-wall-fd", case=case, storage=storage)
+    shape = (7, 1, 6) if case == "ywalls-ny1" else (8, 7, 6)
+    T, K, tau = 25, 6, 0.7
+    u0, F, s, ut = wall_problem(shape, 5)
+    kw = dict(LID if case == "lid" else YWALLS, brinkman=True, checkpoint_every=K)
+    solver = ag.DifferentiableLBM(shape, tau, T, storage=storage, **kw)
+    a, b, c = (torch.tensor(v, device="cuda", requires_grad=True) for v in (u0, F, s))
+    utt = torch.tensor(ut, device="cuda")
+    ag.velocity_matching_loss(solver(a, b, c), utt).backward()
+    gu, gF, gs = (t.grad.double().cpu().numpy() for t in (a, b, c))
+    fwd = ag.DifferentiableLBM(shape, tau, T, storage="f64", **kw)
+
+    def Lat(x, y, z):
+        u = fwd.forward_raw(*(torch.tensor(v, device="cuda") for v in (x, y, z)))
+        return float(ag.velocity_matching_loss(u, utt))
+
+    rng = np.random.default_rng(12)
+    eps = 1e-5 if storage == "f64" else 1e-3
+    fds, ads = [], []
+    for _ in range(4):
+        du = rng.standard_normal(u0.shape) * 0.1
+        dF = rng.standard_normal(F.shape) * 1e-3
+        ds = rng.standard_normal(s.shape)
+        fds.append((Lat(u0 + eps * du, F + eps * dF, s + eps * ds) - Lat(u0 - eps * du, F - eps * dF, s - eps * ds)) / (2 * eps))
+        ads.append(float(np.sum(gu * du) + np.sum(gF * dF) + np.sum(gs * ds)))
+    scale = max(abs(v) for v in fds)
+    for k, (ad, fd) in enumerate(zip(ads, fds)):
+        assert abs(ad - fd) <= tol * scale, (k, ad, fd)
+
+
+def test_ywalls_adjoint_with_fp16_checkpoints(mods):
+    """The y-wall chain's f^0 slot also round-trips through FP16 checkpoints."""
+    torch, ag = mods
+    print_case("adjoint-ywalls-fp16-ckpt")
+    shape = (8, 7, 6)
+    u0, F, s, ut = wall_problem(shape, 6)
+    grads = []
+    for cks in (None, "f16"):
+        solver = ag.DifferentiableLBM(shape, 0.7, 20, checkpoint_every=6, storage="f32", brinkman=True,
+                                      checkpoint_storage=cks, **YWALLS)
+        a, b, c = (torch.tensor(v, device="cuda", requires_grad=True) for v in (u0, F, s))
+        ag.velocity_matching_loss(solver(a, b, c), torch.tensor(ut, device="cuda")).backward()
+        grads.append([t.grad.double().cpu().numpy() for t in (a, b, c)])
+    for g32, g16 in zip(*grads):
+        assert field_scaled_err(g16, g32) < 1e-3

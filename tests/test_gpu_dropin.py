@@ -110,3 +110,19 @@ def test_shape_errors(R):
         R.bgk_step(np.zeros((18, 2, 2, 2)), 0.7)
     with pytest.raises(ValueError):
         R.bgk_step(np.zeros((19, 2, 2, 2)), 0.7, np.zeros((3, 2, 2, 3)))
+
+
+def test_equilibrium_density_moment_pbt(R):
+    """NP/tests/test_pbt_invariants.py:18-20 on the GPU drop-ins (50 Hypothesis samples)."""
+    from lattice_boltzmann_d3q19_amd.invariants import equilibrium_density_moment
+
+    print_case("dropin-pbt-density")
+    equilibrium_density_moment()
+
+
+def test_equilibrium_momentum_moment_pbt(R):
+    """NP/tests/test_pbt_invariants.py:23-25 on the GPU drop-ins (50 Hypothesis samples)."""
+    from lattice_boltzmann_d3q19_amd.invariants import equilibrium_momentum_moment
+
+    print_case("dropin-pbt-momentum")
+    equilibrium_momentum_moment()
