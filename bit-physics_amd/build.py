@@ -62,6 +62,47 @@ def build_node_addon(verbose: bool = False) -> Path | None:
     return NODE_ADDON
 
 
+CPP_DIR = HERE / "cpp"
+INCLUDE = HERE.parent / "include"
+ME_LIB = LIB_DIR / "libd3q19_me.so"
+BIN_DIR = HERE / "bin"
+ROCM_LIB = Path("/opt/rocm/lib")
+
+
+def cxx() -> str:
+    exe = shutil.which("g++")
+    if exe is None:
+        raise RuntimeError("g++ not found (needed for the C++ interface libd3q19_me.so)")
+    return exe
+
+
+def cpp_link_flags(rpath: str) -> list[str]:
+    """Link flags of a program or library over libd3q19 (the HIP runtime it
+    needs is resolved through /opt/rocm/lib at link time)."""
+    return [f"-L{LIB_DIR}", "-ld3q19", f"-L{ROCM_LIB}", f"-Wl,-rpath-link,{ROCM_LIB}", f"-Wl,-rpath,{rpath}"]
+
+
+def build_cpp_me(verbose: bool = False) -> Path:
+    """libd3q19_me.so (the moment-encoded solver's C++ interface over the C
+    ABI) and its capture tool bin/lbm_me_capture."""
+    common = ["-std=c++17", "-O2", "-Wall", "-Wextra", f"-I{INCLUDE}", f"-I{CPP_DIR}"]
+    tmp = ME_LIB.with_suffix(".so.tmp")
+    cmd = [cxx(), *common, "-shared", "-fPIC", str(CPP_DIR / "lbm_me.cpp"), str(CPP_DIR / "capture_npz.cpp"),
+           "-o", str(tmp), *cpp_link_flags("$ORIGIN")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, ME_LIB)
+    BIN_DIR.mkdir(exist_ok=True)
+    tool = BIN_DIR / "lbm_me_capture"
+    cmd = [cxx(), *common, str(CPP_DIR / "lbm_me_capture_main.cpp"), "-o", str(tool), f"-L{LIB_DIR}", "-ld3q19_me",
+           *cpp_link_flags("$ORIGIN/../lib")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return ME_LIB
+
+
 def build(verbose: bool = False) -> Path:
     LIB_DIR.mkdir(exist_ok=True)
     srcs = [str(CSRC / s) for s in SOURCES if (CSRC / s).exists()]
@@ -72,6 +113,7 @@ def build(verbose: bool = False) -> Path:
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     build_node_addon(verbose)
+    build_cpp_me(verbose)
     return LIB
 
 
