@@ -126,3 +126,19 @@ def test_equilibrium_momentum_moment_pbt(R):
 
     print_case("dropin-pbt-momentum")
     equilibrium_momentum_moment()
+
+
+def test_collision_step_run_to_run_bit_exact(R):
+    """The seeded input of the ports' determinism test (stack-e
+    tests/test_determinism.py:56-73: 16x8x3, default_rng(42), rho = 1 + 0.01 N,
+    u = 0.01 N, f = feq_field, one bgk_step at tau 0.7): two runs equal bit for
+    bit, and equal to the oracle bit for bit."""
+    print_case("dropin-determinism-seeded")
+    rng = np.random.default_rng(42)
+    rho0 = 1.0 + 0.01 * rng.standard_normal((16, 8, 3))
+    u0 = 0.01 * rng.standard_normal((3, 16, 8, 3))
+    a = R.bgk_step(R.feq_field(rho0, u0), 0.7)
+    b = R.bgk_step(R.feq_field(rho0, u0), 0.7)
+    assert a.flags.c_contiguous and a.dtype == np.float64
+    assert np.array_equal(a, b)
+    assert np.array_equal(a, oracle.bgk_step(oracle.feq_field(rho0, u0), 0.7))
