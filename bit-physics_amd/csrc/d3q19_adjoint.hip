@@ -50,7 +50,7 @@ __device__ __forceinline__ bool crosses_wall(int cx, int cy, int cz, uint32_t x,
 
 template <class C>
 __device__ __forceinline__ C feq_dev(int i, const C* u0, uint32_t N, uint32_t c) {
-    const C ux = u0[c], uy = u0[N + c], uz = u0[2 * N + c];
+    const C ux = u0[c], uy = u0[(size_t)N + c], uz = u0[2 * (size_t)N + c];
     const C cu = C(CX[i]) * ux + C(CY[i]) * uy + C(CZ[i]) * uz;
     return wgt<C>(i) * (C(3) * cu + C(4.5) * cu * cu - C(1.5) * (ux * ux + uy * uy + uz * uz));
 }
@@ -107,9 +107,9 @@ __global__ void __launch_bounds__(step_block<Tag>())
     const C ir = C(1) / rho;
     const uint32_t N = p.ncell;
     const C s = BRINK ? p.brink[cell] : C(0);
-    const C fx = p.force[cell] - s * mx, fy = p.force[N + cell] - s * my, fz = p.force[2 * N + cell] - s * mz;
+    const C fx = p.force[cell] - s * mx, fy = p.force[(size_t)N + cell] - s * my, fz = p.force[2 * (size_t)N + cell] - s * mz;
     const C ux = (mx + C(0.5) * fx) * ir, uy = (my + C(0.5) * fy) * ir, uz = (mz + C(0.5) * fz) * ir;
-    const C gx = g[cell], gy = g[N + cell], gz = g[2 * N + cell];
+    const C gx = g[cell], gy = g[(size_t)N + cell], gz = g[2 * (size_t)N + cell];
     const C gu = gx * ux + gy * uy + gz * uz;
     const C keep = C(1) - C(0.5) * s;
     C l[Q];
@@ -117,8 +117,8 @@ __global__ void __launch_bounds__(step_block<Tag>())
     for (int j = 0; j < Q; ++j) l[j] = ((C(CX[j]) * gx + C(CY[j]) * gy + C(CZ[j]) * gz) * keep - gu) * ir;
     store_local<Tag>(lam, p, x, y, z, l);
     gF[cell] = C(0.5) * gx * ir;
-    gF[N + cell] = C(0.5) * gy * ir;
-    gF[2 * N + cell] = C(0.5) * gz * ir;
+    gF[(size_t)N + cell] = C(0.5) * gy * ir;
+    gF[2 * (size_t)N + cell] = C(0.5) * gz * ir;
     if (BRINK) gs[cell] = -C(0.5) * (gx * mx + gy * my + gz * mz) * ir;
 }
 
@@ -161,7 +161,7 @@ __global__ void __launch_bounds__(step_block<Tag>())
     const C ir = C(1) / rho;
     const C s = BRINK ? p.brink[cell] : C(0);
     const C m[3] = {mx, my, mz};
-    const C F[3] = {p.force[cell] - s * mx, p.force[N + cell] - s * my, p.force[2 * N + cell] - s * mz};
+    const C F[3] = {p.force[cell] - s * mx, p.force[(size_t)N + cell] - s * my, p.force[2 * (size_t)N + cell] - s * mz};
     const C u[3] = {(mx + C(0.5) * F[0]) * ir, (my + C(0.5) * F[1]) * ir, (mz + C(0.5) * F[2]) * ir};
     const C usq = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
     const C om = p.omega, pref = p.pref;
@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(step_block<Tag>())
     const uint32_t N = p.ncell;
     C l[Q];
     load_local<Tag>(lam, p, x, y, z, l);
-    const C u[3] = {u0[cell], u0[N + cell], u0[2 * N + cell]};
+    const C u[3] = {u0[cell], u0[(size_t)N + cell], u0[2 * (size_t)N + cell]};
     C g[3] = {C(0), C(0), C(0)};
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -266,8 +266,8 @@ __global__ void __launch_bounds__(step_block<Tag>())
         for (int a = 0; a < 3; ++a) g[a] += sw * (C(3) * c[a] + C(9) * cu * c[a] - C(3) * u[a]);
     }
     gu0[cell] = g[0];
-    gu0[N + cell] = g[1];
-    gu0[2 * N + cell] = g[2];
+    gu0[(size_t)N + cell] = g[1];
+    gu0[2 * (size_t)N + cell] = g[2];
 }
 
 // Re-encode a state between storage formats (checkpoint compression).

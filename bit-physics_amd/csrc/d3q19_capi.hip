@@ -303,6 +303,28 @@ int launch_init(d3q19_lattice* L, const In* f_dev) {
     return D3Q19_OK;
 }
 
+// feq(rho, u) straight into the lattice storage (all storages but ME16, whose
+// codes are made from populations).
+template <class Tag, class In>
+int launch_init_eq(d3q19_lattice* L, const In* rho_dev, const In* u_dev) {
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        return fail(D3Q19_EINVAL, "D3Q19_ME16 equilibrium is loaded through populations");
+    } else {
+        typedef typename Codec<Tag>::C C;
+        const Params<C> p = make_params<C>(L);
+        const StepGrid sg = step_grid<Tag>(L);
+        if (L->rows)
+            k_load_eq<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
+        else
+            k_load_eq<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
+        HIPCHK(hipGetLastError());
+        L->has_state = true;
+        L->local = true;
+        L->steps = 0;
+        return D3Q19_OK;
+    }
+}
+
 template <class Tag, class Out, bool ROWS>
 void launch_readback_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p, Out* f,
                        Out* rho, Out* u, const typename Codec<Tag>::raw* in) {
@@ -617,7 +639,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     // lattice and its codes in the same layout at 2 bytes per element
     const size_t es = me ? 4 : storage_bytes(cfg->storage);
     if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16, BF16, BF16SR or ME16");
-    if (ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (ncell * 8 must fit 32 bits)");
+    if (ncell >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (nx * ny * nz must be < 2^32)");
     if (cfg->nx > 65535 || cfg->ny > 65535) return fail(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (!(cfg->tau > 0.5)) return fail(D3Q19_EINVAL, "tau must be > 0.5");
     if (cfg->boundary < D3Q19_BC_PERIODIC || cfg->boundary > D3Q19_BC_HALFWAY)
@@ -663,6 +685,15 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
         return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
     }
     L->bytes = buf0_bytes + lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
+    // Row padding (z >= nz of a padded row) is never written by a kernel:
+    // zero it once, so that the raw state (read_state, checkpoints) is a
+    // function of the lattice's content alone.
+    if (L->rows && L->nzp != (uint32_t)cfg->nz) {
+        if (hipMemset(L->buf[0], 0, buf0_bytes) != hipSuccess || hipMemset(L->buf[1], 0, lat_bytes) != hipSuccess) {
+            d3q19_destroy(L);
+            return fail(D3Q19_EHIP, "clearing the lattice padding failed");
+        }
+    }
     *out = L;
     return D3Q19_OK;
 }
@@ -727,6 +758,15 @@ int d3q19_set_equilibrium(d3q19_lattice* L, const void* rho, const void* u, int3
             prho = drho.p;
         }
     }
+    if (!L->me) {
+        const int rc = dt == D3Q19_F64
+                           ? D3Q19_DISPATCH(L, launch_init_eq, L, (const double*)prho, (const double*)pu)
+                           : D3Q19_DISPATCH(L, launch_init_eq, L, (const float*)prho, (const float*)pu);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(L->stream));
+        return D3Q19_OK;
+    }
+    // ME16: populations first, then their codes
     if (df.alloc((size_t)Q * nc * es) != hipSuccess) return fail(D3Q19_ENOMEM, "staging allocation failed");
     if (dt == D3Q19_F64)
         k_fn_feq<double><<<nblocks(nc), BLOCK, 0, L->stream>>>((const double*)prho, (const double*)pu,
@@ -1011,7 +1051,7 @@ size_t d3q19_device_bytes(const d3q19_lattice* L) { return L ? L->bytes : 0; }
 // ---------------------------------------------------------------------------
 static int check_fn(const void* a, const void* b, int64_t ncell, int32_t dt) {
     if (!a || !b) return fail(D3Q19_EINVAL, "null array");
-    if (ncell < 1 || (uint64_t)ncell * 8 >= (1ull << 32)) return fail(D3Q19_EINVAL, "bad cell count");
+    if (ncell < 1 || (uint64_t)ncell >= (1ull << 32)) return fail(D3Q19_EINVAL, "bad cell count (1 .. 2^32 - 1)");
     return check_dtype(dt);
 }
 

@@ -441,8 +441,8 @@ __device__ __forceinline__ void cell_force(const Params<C>& p, uint32_t cell, C 
         fz = p.fu[2];
     } else if (p.force_mode == FORCE_FIELD) {
         fx = p.force[cell];
-        fy = p.force[p.ncell + cell];
-        fz = p.force[2 * p.ncell + cell];
+        fy = p.force[(size_t)p.ncell + cell];
+        fz = p.force[2 * (size_t)p.ncell + cell];
     }
     if (p.brink) {
         const C s = p.brink[cell];

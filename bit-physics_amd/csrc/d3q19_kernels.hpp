@@ -33,8 +33,8 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;
-        u_out[p.ncell + cell] = (float)uy;
-        u_out[2 * p.ncell + cell] = (float)uz;
+        u_out[(size_t)p.ncell + cell] = (float)uy;
+        u_out[2 * (size_t)p.ncell + cell] = (float)uz;
     }
 }
 
@@ -78,8 +78,8 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;
-        u_out[p.ncell + cell] = (float)uy;
-        u_out[2 * p.ncell + cell] = (float)uz;
+        u_out[(size_t)p.ncell + cell] = (float)uy;
+        u_out[2 * (size_t)p.ncell + cell] = (float)uz;
     }
 }
 
@@ -123,8 +123,8 @@ static __global__ void __launch_bounds__(step_block<Tag>())
         if (rho_out) rho_out[cell] = (Out)rho;
         if (u_out) {
             u_out[cell] = (Out)ux;
-            u_out[p.ncell + cell] = (Out)uy;
-            u_out[2 * p.ncell + cell] = (Out)uz;
+            u_out[(size_t)p.ncell + cell] = (Out)uy;
+            u_out[2 * (size_t)p.ncell + cell] = (Out)uz;
         }
     }
 }
@@ -414,7 +414,7 @@ static __global__ void __launch_bounds__(BLOCK)
     T fx = T(0), fy = T(0), fz = T(0);
     if (force) {
         fx = force[cell];
-        fy = force[ncell + cell];
+        fy = force[(size_t)ncell + cell];
         fz = force[2 * (size_t)ncell + cell];
     }
     ref_collide(fc, tau, force != nullptr, fx, fy, fz);
@@ -489,11 +489,37 @@ static __global__ void __launch_bounds__(BLOCK)
     const uint32_t cell = blockIdx.x * BLOCK + threadIdx.x;
     if (cell >= ncell) return;
     const RefConst<T> k = ref_const<T>();
-    const T ux = u[cell], uy = u[ncell + cell], uz = u[2 * (size_t)ncell + cell];
+    const T ux = u[cell], uy = u[(size_t)ncell + cell], uz = u[2 * (size_t)ncell + cell];
     const T u_sq = ux * ux + uy * uy + uz * uz;
     const T r = rho ? rho[cell] : T(1);
 #pragma unroll
     for (int i = 0; i < Q; ++i) out[(size_t)i * ncell + cell] = ref_feq(i, r, ux, uy, uz, u_sq, k);
+}
+
+// set_equilibrium in one pass: the state k_fn_feq followed by k_load would
+// give (each f_i^eq rounded to In, then f_i - w_i), with no (19, ncell)
+// staging array -- which for the largest lattices would not fit beside them.
+template <class Tag, class In, bool ROWS = true>
+static __global__ void __launch_bounds__(step_block<Tag>())
+    k_load_eq(const In* __restrict__ rho, const In* __restrict__ u, typename Codec<Tag>::raw* __restrict__ out,
+              Params<typename Codec<Tag>::C> p) {
+#pragma clang fp contract(off)
+    typedef typename Codec<Tag>::C C;
+    uint32_t x, y, z;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
+    const uint32_t cell = (x * p.ny + y) * p.nz + z;
+    const bool solid = p.solid && p.solid[cell];
+    const RefConst<In> k = ref_const<In>();
+    const In ux = u[cell], uy = u[(size_t)p.ncell + cell], uz = u[2 * (size_t)p.ncell + cell];
+    const In u_sq = ux * ux + uy * uy + uz * uz;
+    const In r = rho ? rho[cell] : In(1);
+    C d[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const In feq = ref_feq(i, r, ux, uy, uz, u_sq, k);
+        d[i] = solid ? C(0) : (C)((double)feq - wgt<double>(i));
+    }
+    store_local<Tag>(out, p, x, y, z, d);
 }
 
 // density_field / momentum_field / macroscopic_velocity; any output may be null.
@@ -512,18 +538,18 @@ static __global__ void __launch_bounds__(BLOCK)
     if (rho_out) rho_out[cell] = r;
     if (mom_out) {
         mom_out[cell] = mx;
-        mom_out[ncell + cell] = my;
+        mom_out[(size_t)ncell + cell] = my;
         mom_out[2 * (size_t)ncell + cell] = mz;
     }
     if (u_out) {
         const T rs = rho_floor(r);
         if (force) {
             mx = mx + T(0.5) * force[cell];
-            my = my + T(0.5) * force[ncell + cell];
+            my = my + T(0.5) * force[(size_t)ncell + cell];
             mz = mz + T(0.5) * force[2 * (size_t)ncell + cell];
         }
         u_out[cell] = mx / rs;
-        u_out[ncell + cell] = my / rs;
+        u_out[(size_t)ncell + cell] = my / rs;
         u_out[2 * (size_t)ncell + cell] = mz / rs;
     }
 }

@@ -178,8 +178,8 @@ static __global__ void __launch_bounds__(256)
     if (rho_out) rho_out[cell] = rho;
     if (u_out) {
         u_out[cell] = ux;
-        u_out[p.ncell + cell] = uy;
-        u_out[2 * p.ncell + cell] = uz;
+        u_out[(size_t)p.ncell + cell] = uy;
+        u_out[2 * (size_t)p.ncell + cell] = uz;
     }
 }
 

@@ -100,7 +100,7 @@ class Lattice:
         for face, vel in (face_velocity or {}).items():
             for a in range(3):
                 cfg.face_velocity[FACES[face]][a] = float(vel[a])
-        self._force_field = None
+        field = None  # copied into the library below; no reference is kept
         self._me_ranges = None
         if force is None:
             cfg.force_mode = FORCE_NONE
@@ -110,15 +110,15 @@ class Lattice:
                 cfg.force[a] = float(force[a])
         else:
             cfg.force_mode = FORCE_FIELD
-            self._force_field = force
+            field = force
         cfg.device = self.device
         self._cfg = cfg
         self._stream = stream
         h = ctypes.c_void_p()
         call("d3q19_create", ctypes.byref(cfg), self._stream_arg(), ctypes.byref(h))
         self._h = h
-        if self._force_field is not None:
-            self.set_force_field(self._force_field)
+        if field is not None:
+            self.set_force_field(field)
 
     # ---- plumbing -------------------------------------------------------
     def _stream_arg(self):
@@ -283,12 +283,10 @@ class Lattice:
         return {"mass": out[0], "momentum": (out[1], out[2], out[3]), "kinetic_energy": out[4]}
 
     # ---- checkpoint / restore -----------------------------------------
-    def checkpoint(self, path):
-        """Write the stored state (the population buffer in its storage format
-        and layout, whether it holds f^n in place, the step count) and, for
-        "me16", the moment ranges to ``path`` (.npz).  Configuration and
-        fields (force field, solids, Brinkman) are not saved: restore into a
-        lattice created and set up the same way."""
+    def raw_state(self):
+        """(bytes, local, step_count) of the stored state: the population
+        buffer in its storage format and layout (uint8 array), whether it holds
+        f^n in place (not yet streamed), and the step count."""
         lib = load()
         raw = np.empty(int(lib.d3q19_state_bytes(self._h)), np.uint8)
         local = ctypes.c_int32()
@@ -296,7 +294,16 @@ class Lattice:
         self._sync_stream()
         call("d3q19_read_state", self._h, ctypes.c_void_p(raw.ctypes.data), ctypes.byref(local),
              ctypes.byref(steps))
-        arrays = {"state": raw, "local": np.int32(local.value), "step_count": np.int64(steps.value),
+        return raw, bool(local.value), int(steps.value)
+
+    def checkpoint(self, path):
+        """Write the stored state (the population buffer in its storage format
+        and layout, whether it holds f^n in place, the step count) and, for
+        "me16", the moment ranges to ``path`` (.npz).  Configuration and
+        fields (force field, solids, Brinkman) are not saved: restore into a
+        lattice created and set up the same way."""
+        raw, local, steps = self.raw_state()
+        arrays = {"state": raw, "local": np.int32(local), "step_count": np.int64(steps),
                   "shape": np.asarray(self.shape, np.int64), "tau": np.float64(self.tau),
                   "storage": np.asarray(self.storage), "boundary": np.asarray(self.boundary)}
         if self._me_ranges is not None:
