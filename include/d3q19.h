@@ -83,14 +83,17 @@ void d3q19_config_default(d3q19_config* cfg);
 /* ---- stateful lattice (the step / read-back path) ---------------------
  * Replaces the runner loop of NP/sim.py:312-326 / :350-364:
  *   f = bgk_step(f, tau, force); f = apply_bounce_back_y_walls(f, top, bottom)
- * with one fused stream-collide kernel per step. */
+ * with one fused stream-collide kernel per step.  A lattice holds at most
+ * 2^32 - 1 cells (nx, ny <= 65535); host arrays passed to it are staged
+ * through device buffers of their size, device arrays are used in place. */
 int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out);
 int d3q19_destroy(d3q19_lattice* lat);
 int d3q19_set_stream(d3q19_lattice* lat, void* stream);
 /* Set the state to populations f (pre-collision, the reference's f^n). */
 int d3q19_set_populations(d3q19_lattice* lat, const void* f, int32_t dtype, int32_t on_device);
 /* Set the state to feq_field(rho, u) (NP/reference/equilibrium.py:71-98,
-   NP/sim.py:281-285).  rho may be NULL for rho = 1. */
+   NP/sim.py:281-285).  rho may be NULL for rho = 1.  One pass, no
+   (19,Nx,Ny,Nz) staging (D3Q19_ME16: through populations). */
 int d3q19_set_equilibrium(d3q19_lattice* lat, const void* rho, const void* u, int32_t dtype,
                           int32_t on_device);
 /* Per-cell body force (3,Nx,Ny,Nz) for D3Q19_FORCE_FIELD (bgk.py:45-49 force_lattice). */
