@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -93,17 +94,20 @@ struct Sha256 {
 };
 
 // ---- CRC-32 (ISO-HDLC, reflected polynomial 0xEDB88320) used by zip --------
-uint32_t crc32(const unsigned char* p, size_t n) {
-    static uint32_t table[256];
-    static bool ready = false;
-    if (!ready) {
+struct CrcTable {
+    uint32_t t[256];
+    CrcTable() {
         for (uint32_t i = 0; i < 256; ++i) {
             uint32_t c = i;
             for (int k = 0; k < 8; ++k) c = (c & 1) ? 0xEDB88320u ^ (c >> 1) : c >> 1;
-            table[i] = c;
+            t[i] = c;
         }
-        ready = true;
     }
+};
+
+uint32_t crc32(const unsigned char* p, size_t n) {
+    static const CrcTable tab;  // initialised once, thread-safe
+    const uint32_t* table = tab.t;
     uint32_t c = 0xFFFFFFFFu;
     for (size_t i = 0; i < n; ++i) c = table[(c ^ p[i]) & 0xFF] ^ (c >> 8);
     return c ^ 0xFFFFFFFFu;
@@ -296,8 +300,12 @@ std::string Json::dump(int indent) const {
         case Int: return std::to_string(i);
         case Num: {
             if (!std::isfinite(d)) throw std::runtime_error("JSON cannot hold a non-finite number");
+            // the shortest of 15, 16, 17 significant digits that reads back exactly
             char t[32];
-            std::snprintf(t, sizeof t, "%.17g", d);
+            for (int prec = 15; prec <= 17; ++prec) {
+                std::snprintf(t, sizeof t, "%.*g", prec, d);
+                if (std::strtod(t, nullptr) == d) break;
+            }
             std::string s = t;
             if (s.find_first_of(".eE") == std::string::npos) s += ".0";
             return s;

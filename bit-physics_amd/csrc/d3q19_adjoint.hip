@@ -545,7 +545,9 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     if (cks != cfg->storage && cks != D3Q19_F16)
         return set_error(D3Q19_EINVAL, "checkpoint storage must be the lattice storage or F16");
     const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
-    if (ncell * 8 >= (1ull << 32)) return set_error(D3Q19_EINVAL, "lattice too large");
+    // four or more full states must fit in memory, so no adjoint that can be
+    // allocated reaches this bound; the (3, ncell) field offsets are 64-bit anyway
+    if (3 * ncell >= (1ull << 32)) return set_error(D3Q19_EINVAL, "lattice too large (3 * ncell must be < 2^32)");
     if (cfg->nx > 65535 || cfg->ny > 65535) return set_error(D3Q19_EINVAL, "nx and ny must be <= 65535");
     if (hipSetDevice(cfg->device) != hipSuccess) return set_error(D3Q19_EHIP, "hipSetDevice failed");
 
