@@ -236,17 +236,21 @@ __device__ __forceinline__ const char* row_of(const void* base, const Params<C>&
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
 }
 
-// Population i of a row at byte offset zoff.  The fused step uses plain
-// loads: the lines that a z-shifted pull straddles are shared with the
-// neighbouring wavefront and stay in L2 (with the nontemporal hint they were
-// re-fetched: 12% more FP32 reads, measured with FETCH_SIZE at 256^3).  The
-// adjoint's 4-byte loads keep the hint (NT): its backward pass measured 4%
-// faster with it.
+// Population i of a row at byte offset zoff; NT: with the nontemporal hint
+// (4- and 2-byte storage).  In the fused step the hint depends on the
+// direction.  A z-shifted pull (c_z != 0) straddles a line that the
+// neighbouring wavefront of the same row also reads: loaded plainly, it stays
+// in L2 (with the hint it was re-fetched, 12% more FP32 reads measured with
+// FETCH_SIZE at 256^3).  A pull with c_z = 0 reads whole lines that no other
+// wavefront touches: the hint keeps them from displacing the shared ones
+// (FP32 step 39.9 against 38.9 GLUPS, FP16 81.1 against 80.0, at 256^3).
+// The adjoint hints all its 4-byte loads: its backward pass measured 4%
+// faster with them.
 template <class Tag, bool NT = false>
 __device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
     typedef typename Codec<Tag>::raw R;
     const R* a = (const R*)(row + (uint32_t)i * dir_bytes + zoff);
-    if constexpr (NT && sizeof(R) == 4)
+    if constexpr (NT && sizeof(R) <= 4)
         return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
     else
         return Codec<Tag>::dec(*a, i);
@@ -382,7 +386,10 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
             const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
-            d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
+            if (CZ[i] == 0)
+                d[i] = ld<Tag, true>(row, i, DB, z * ES);
+            else
+                d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
         }
     }
     if constexpr (BC == BC_YWALLS) ywall_swap<C>(p, y, d);
@@ -399,6 +406,8 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
                 const uint32_t sx = xs[1 - CX[i]], sy = ys[1 - CY[i]], sz = zs[1 - CZ[i]];
                 if (p.solid && p.solid[(sx * p.ny + sy) * p.nz + sz])
                     d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES);
+                else if (CZ[i] == 0)
+                    d[i] = ld<Tag, true>(row_of(in, p, sx, sy), i, DB, z * ES);
                 else
                     d[i] = ld<Tag, NT>(row_of(in, p, sx, sy), i, DB, sz * ES);
             }
