@@ -247,13 +247,18 @@ __device__ __forceinline__ const char* row_of(const void* base, const Params<C>&
 // The adjoint hints all its 4-byte loads: its backward pass measured 4%
 // faster with them.
 template <class Tag, bool NT = false>
-__device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
+__device__ __forceinline__ typename Codec<Tag>::C ld_at(const char* addr, int i) {
     typedef typename Codec<Tag>::raw R;
-    const R* a = (const R*)(row + (uint32_t)i * dir_bytes + zoff);
+    const R* a = (const R*)addr;
     if constexpr (NT && sizeof(R) <= 4)
         return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
     else
         return Codec<Tag>::dec(*a, i);
+}
+
+template <class Tag, bool NT = false>
+__device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
+    return ld_at<Tag, NT>(row + (uint32_t)i * dir_bytes + zoff, i);
 }
 
 // Stores are nontemporal: each population is written once per step and read
@@ -396,21 +401,58 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
     if constexpr (BC == BC_HALFWAY) {
         const uint32_t wm = p.wall_mask;
         const char* self = row_of(in, p, x, y);
+        // Solid source cells: all flags are loaded before any population, so
+        // that no population load waits on a flag (none when there are no
+        // solids: the branch is uniform).
+        bool solid_src[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) solid_src[i] = false;
+        if (p.solid) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i)
+                solid_src[i] = p.solid[(xs[1 - CX[i]] * p.ny + ys[1 - CY[i]]) * p.nz + zs[1 - CZ[i]]] != 0;
+        }
 #pragma unroll
         for (int i = 0; i < Q; ++i) {
-            const int face = halfway_face(i, wm, x, y, z, p);
-            if (face >= 0) {
-                const C cu = cdot<C>(i, p.face_u[face][0], p.face_u[face][1], p.face_u[face][2]);
-                d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES) + C(6) * wgt<C>(i) * cu;
-            } else {
-                const uint32_t sx = xs[1 - CX[i]], sy = ys[1 - CY[i]], sz = zs[1 - CZ[i]];
-                if (p.solid && p.solid[(sx * p.ny + sy) * p.nz + sz])
-                    d[i] = ld<Tag, NT>(self, OPP[i], DB, z * ES);
-                else if (CZ[i] == 0)
-                    d[i] = ld<Tag, true>(row_of(in, p, sx, sy), i, DB, z * ES);
-                else
-                    d[i] = ld<Tag, NT>(row_of(in, p, sx, sy), i, DB, sz * ES);
+            // The pull crosses a wall face; the face that counts for the
+            // moving-wall term is the last in face order (z over y over x,
+            // as halfway_face), with compile-time face indices.
+            const bool wx = (CX[i] > 0 && x == 0 && (wm & 1u)) || (CX[i] < 0 && x + 1 == p.nx && (wm & 2u));
+            const bool wy = (CY[i] > 0 && y == 0 && (wm & 4u)) || (CY[i] < 0 && y + 1 == p.ny && (wm & 8u));
+            const bool wz = (CZ[i] > 0 && z == 0 && (wm & 16u)) || (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u));
+            const bool wall = wx || wy || wz;
+            C u0 = C(0), u1 = C(0), u2 = C(0);
+            if (CX[i] != 0) {
+                const int f = CX[i] > 0 ? 0 : 1;
+                u0 = wx ? p.face_u[f][0] : u0;
+                u1 = wx ? p.face_u[f][1] : u1;
+                u2 = wx ? p.face_u[f][2] : u2;
             }
+            if (CY[i] != 0) {
+                const int f = CY[i] > 0 ? 2 : 3;
+                u0 = wy ? p.face_u[f][0] : u0;
+                u1 = wy ? p.face_u[f][1] : u1;
+                u2 = wy ? p.face_u[f][2] : u2;
+            }
+            if (CZ[i] != 0) {
+                const int f = CZ[i] > 0 ? 4 : 5;
+                u0 = wz ? p.face_u[f][0] : u0;
+                u1 = wz ? p.face_u[f][1] : u1;
+                u2 = wz ? p.face_u[f][2] : u2;
+            }
+            // Bounce-back (a wall face, or a solid source cell) reads the
+            // opposite population of the cell itself, else the pull source.
+            // One load per direction from a selected address: a load in one
+            // arm of a branch would have to complete (and, for 16-bit
+            // storage, be decoded) before the arms join, serialising the 19
+            // loads.  opp(i) has the weight class of i, so the value decodes
+            // with index i either way.
+            const bool back = wall || solid_src[i];
+            const char* src = back ? self + (uint32_t)OPP[i] * DB + z * ES
+                                   : row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]) + (uint32_t)i * DB +
+                                         zs[1 - CZ[i]] * ES;
+            const C v = CZ[i] == 0 ? ld_at<Tag, true>(src, i) : ld_at<Tag, NT>(src, i);
+            d[i] = wall ? v + C(6) * wgt<C>(i) * cdot<C>(i, u0, u1, u2) : v;
         }
     }
 }
