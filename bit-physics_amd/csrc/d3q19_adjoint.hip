@@ -145,10 +145,13 @@ __global__ void __launch_bounds__(step_block<Tag>())
     C mu[Q];
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        if (BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p))
-            mu[i] = ld<Tag, true>(self_row, OPP[i], p.dir_bytes, z * ES);
-        else
-            mu[i] = ld<Tag, true>(row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]), i, p.dir_bytes, zs[1 + CZ[i]] * ES);
+        // one load per direction from a selected address (no load inside a
+        // branch arm, whose join would wait for every load in flight)
+        const bool back = BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p);
+        const char* src = back ? self_row + (uint32_t)OPP[i] * p.dir_bytes + z * ES
+                               : row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]) + (uint32_t)i * p.dir_bytes +
+                                     zs[1 + CZ[i]] * ES;
+        mu[i] = ld_at<Tag, true>(src, i);
     }
     C d[Q];
     if constexpr (LOCAL0)
