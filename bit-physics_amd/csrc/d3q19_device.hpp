@@ -236,21 +236,22 @@ __device__ __forceinline__ const char* row_of(const void* base, const Params<C>&
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
 }
 
-// Population i of a row at byte offset zoff; NT: with the nontemporal hint
-// (4- and 2-byte storage).  In the fused step the hint depends on the
+// Population i of a row at byte offset zoff; NT: with the nontemporal hint.
+// In the fused step the hint depends on the
 // direction.  A z-shifted pull (c_z != 0) straddles a line that the
 // neighbouring wavefront of the same row also reads: loaded plainly, it stays
 // in L2 (with the hint it was re-fetched, 12% more FP32 reads measured with
 // FETCH_SIZE at 256^3).  A pull with c_z = 0 reads whole lines that no other
 // wavefront touches: the hint keeps them from displacing the shared ones
-// (FP32 step 39.9 against 38.9 GLUPS, FP16 81.1 against 80.0, at 256^3).
+// (256^3 step: FP32 39.9 against 38.9 GLUPS, FP16 81.1 against 80.0, FP64
+// 18.4 against 17.9).
 // The adjoint hints all its 4-byte loads: its backward pass measured 4%
 // faster with them.
 template <class Tag, bool NT = false>
 __device__ __forceinline__ typename Codec<Tag>::C ld_at(const char* addr, int i) {
     typedef typename Codec<Tag>::raw R;
     const R* a = (const R*)addr;
-    if constexpr (NT && sizeof(R) <= 4)
+    if constexpr (NT)
         return Codec<Tag>::dec(__builtin_nontemporal_load(a), i);
     else
         return Codec<Tag>::dec(*a, i);
