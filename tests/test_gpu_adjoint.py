@@ -39,7 +39,7 @@ def gpu_grad(torch, ag, shape, tau, T, K, storage, u0, F, ut):
     utt = torch.tensor(ut, device="cuda")
     loss = ag.velocity_matching_loss(solver(u0t, Ft), utt)
     loss.backward()
-    return float(loss), u0t.grad.cpu().numpy(), Ft.grad.cpu().numpy(), solver
+    return float(loss.detach()), u0t.grad.cpu().numpy(), Ft.grad.cpu().numpy(), solver
 
 
 @pytest.mark.parametrize("T,K", [(7, 3), (6, 6), (5, 1)])
