@@ -330,3 +330,32 @@ def test_bf16_stochastic_rounding_is_unbiased(amd):
     ulp = float(hi) - float(lo)
     assert abs(means["bf16"] - float(lo)) < 1e-6 * ulp
     assert abs(means["bf16sr"] - d0) < 0.01 * ulp
+
+
+@pytest.mark.parametrize("storage,sizes", [("f32", (32, 64, 128)), ("f64", (32, 64))])
+def test_taylor_green_low_re_decay_is_second_order(amd, storage, sizes):
+    """Taylor-Green at Re ~ 0.1 (u0 = 1e-3, tau 0.8, nu = 0.1): KE(t)/KE(0) follows
+    exp(-6 nu k^2 t), k = 2 pi / N, to the BGK discretisation error, which falls
+    as 1 / N^2.  Measured with the FP64 oracle: relative error x N^2 = 18.8 at
+    N = 16, 32, 64 (KE decays slightly faster than the law).  The band [15, 23]
+    pins the relaxation-time / viscosity relation and second-order accuracy; an
+    error in the collision or streaming would leave it."""
+    from lattice_boltzmann_d3q19_amd.initial import taylor_green, taylor_green_ke_decay
+
+    tau = 0.8
+    nu = (tau - 0.5) / 3.0
+    for n in sizes:
+        print_case("tg-low-re-decay", storage=storage, n=n)
+        shape = (n, n, n)
+        rho, u = taylor_green(shape, 1e-3)
+        steps = int(round(1.0 / (6.0 * nu * (2 * np.pi / n) ** 2)))
+        lat = amd.Lattice(shape, tau, storage=storage).set_equilibrium(rho, u)
+        ke0 = lat.diagnostics()["kinetic_energy"]
+        lat.step(steps)
+        ratio = lat.diagnostics()["kinetic_energy"] / ke0
+        lat.close()
+        law = taylor_green_ke_decay(shape, nu, steps)
+        err = abs(ratio / law - 1.0)
+        print(f"N={n} steps={steps} KE ratio {ratio:.6f} law {law:.6f} err*N^2 {err * n * n:.2f}", flush=True)
+        assert ratio < law
+        assert 15.0 < err * n * n < 23.0, (n, err)
