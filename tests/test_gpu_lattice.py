@@ -359,3 +359,34 @@ def test_taylor_green_low_re_decay_is_second_order(amd, storage, sizes):
         print(f"N={n} steps={steps} KE ratio {ratio:.6f} law {law:.6f} err*N^2 {err * n * n:.2f}", flush=True)
         assert ratio < law
         assert 15.0 < err * n * n < 23.0, (n, err)
+
+
+@pytest.mark.parametrize("storage,tol", [("f64", 1e-9), ("f32", 2e-5)])
+def test_halfway_poiseuille_and_couette_analytic(amd, storage, tol):
+    """Half-way walls on -y/+y.  With a uniform Guo force g the steady profile
+    is the parabola u(y) = g (y + 1/2)(H - 1/2 - y) / (2 nu), exact for BGK at
+    tau = 1/2 + sqrt(3/16); with the +y wall moving at U and no force, the
+    linear u(y) = U (y + 1/2) / H.  Analytic pins of the half-way mode on the
+    GPU (the oracle's own tests pin the same closed forms)."""
+    import math
+
+    print_case("halfway-analytic", storage=storage)
+    nx, ny, nz = 2, 12, 2
+    tau = 0.5 + math.sqrt(3.0 / 16.0)
+    nu = (tau - 0.5) / 3.0
+    g = 1e-6
+    lat = amd.Lattice((nx, ny, nz), tau, storage=storage, boundary="halfway", wall_faces=("-y", "+y"),
+                      force=(g, 0.0, 0.0)).set_equilibrium()
+    lat.step(20000)
+    _, u = lat.moments()
+    y = np.arange(ny)
+    ua = g / (2 * nu) * (y + 0.5) * (ny - 0.5 - y)
+    assert np.max(np.abs(u[0, 0, :, 0] - ua)) / ua.max() < tol
+    assert np.max(np.abs(u[0] - u[0, :1, :, :1])) == 0.0  # x and z invariant
+    U = 0.01
+    lat = amd.Lattice((nx, 10, nz), 0.8, storage=storage, boundary="halfway", wall_faces=("-y", "+y"),
+                      face_velocity={"+y": (U, 0.0, 0.0)}).set_equilibrium()
+    lat.step(12000)
+    _, u = lat.moments()
+    ua = U * (np.arange(10) + 0.5) / 10
+    assert np.max(np.abs(u[0, 0, :, 0] - ua)) / U < tol
