@@ -358,6 +358,56 @@ def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     }
 
 
+def config_baselines(amd, torch):
+    """CPU (FP64 oracle, OpenMP) MLUPS beside the GPU GLUPS (FP32 storage) of the
+    same workload, for the cavity (C1, full run) and turbulence (C3, 5 steps) configs."""
+    import numpy as np
+
+    import oracle
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+
+    def gpu_rate(lat, steps):
+        lat.step(5)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        lat.step(steps)
+        e1.record()
+        torch.cuda.synchronize()
+        rate = math.prod(lat.shape) * steps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+        lat.close()
+        return rate
+
+    out = {}
+    # C1: 64^3 lid-driven cavity, half-way walls on all six faces, lid (0.05, 0, 0) on z = Nz - 1, tau 0.6
+    shape, tau, steps = (64, 64, 64), 0.6, 1000
+    fu = np.zeros((6, 3))
+    fu[5] = (0.05, 0.0, 0.0)
+    f0 = oracle.feq_field(np.ones(shape), np.zeros((3,) + shape))
+    t = time.perf_counter()
+    oracle.run_halfway(f0, tau, steps, wall_mask=63, face_velocity=fu)
+    dt = time.perf_counter() - t
+    lat = amd.Lattice(shape, tau, storage="f32", boundary="halfway",
+                      wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                      face_velocity={"+z": (0.05, 0.0, 0.0)}).set_equilibrium()
+    out["C1-cavity-64^3-1000steps"] = {"cpu_MLUPS": math.prod(shape) * steps / dt / 1e6, "cpu_s": round(dt, 2),
+                                       "gpu_GLUPS_f32": gpu_rate(lat, steps)}
+    # C3: 384^3 decaying turbulence (random-phase solenoidal, rms 0.03), tau 0.51, 5 CPU steps
+    shape, tau = (384, 384, 384), 0.51
+    u = random_solenoidal(shape, 0.03, seed=42)
+    f0 = oracle.feq_field(np.ones(shape), u)
+    t = time.perf_counter()
+    oracle.run_reference(f0, tau, 5)
+    dt = time.perf_counter() - t
+    del f0
+    lat = amd.Lattice(shape, tau, storage="f32").set_equilibrium(None, torch.tensor(u, device="cuda",
+                                                                                      dtype=torch.float32))
+    del u
+    out["C3-turbulence-384^3-5steps"] = {"cpu_MLUPS": math.prod(shape) * 5 / dt / 1e6, "cpu_s": round(dt, 2),
+                                         "gpu_GLUPS_f32": gpu_rate(lat, 200)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,7 +416,7 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--tau", type=float, default=0.55)
     ap.add_argument("--u0", type=float, default=0.04)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-me16", action="store_true")
@@ -489,6 +539,10 @@ def main() -> int:
 
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(shape, args.tau, rho0, u0, args.cpu_steps)
+        result["cpu_baseline"]["configs"] = config_baselines(amd, torch)
+        result["cpu_baseline"]["reference_published_MLUPS"] = {
+            "range": [1.1, 4.3], "posture": "single thread, FP64, capture included, i7-12700KF",
+            "source": "docs/perf-ledger.md:27-42 of the reference"}
 
     print(json.dumps(result), flush=True)
     return 0
