@@ -17,7 +17,7 @@ HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB_DIR = HERE / "lib"
 LIB = LIB_DIR / "libd3q19.so"
-SOURCES = ["d3q19_capi.hip", "d3q19_adjoint.hip", "d3q19_probe.hip"]
+SOURCES = ["d3q19_capi.hip", "d3q19_adjoint.hip", "d3q19_calibration.hip"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

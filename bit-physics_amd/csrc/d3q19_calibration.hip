@@ -1,4 +1,4 @@
-// d3q19_probe.hip -- calibration copy kernel: a plain 16-byte-per-lane device
+// d3q19_calibration.hip -- calibration copy kernel: a plain 16-byte-per-lane device
 // copy of known size, used to measure the achievable HBM bandwidth and to
 // scale the FETCH_SIZE / WRITE_SIZE counters against a known byte count.
 #include <hip/hip_runtime.h>
