@@ -202,7 +202,7 @@ NpyArray npy_f64(const std::vector<double>& v, std::vector<int64_t> shape) {
     a.descr = "<f8";
     a.shape = std::move(shape);
     a.bytes.resize(v.size() * sizeof(double));
-    std::memcpy(a.bytes.data(), v.data(), a.bytes.size());
+    if (!v.empty()) std::memcpy(a.bytes.data(), v.data(), a.bytes.size());
     return a;
 }
 
