@@ -371,6 +371,23 @@ __device__ __forceinline__ int halfway_face(int i, uint32_t wm, uint32_t x, uint
     return face;
 }
 
+// Periodic pull of all 19 directions (the load hints as described at ld).
+template <class Tag, bool NT>
+__device__ __forceinline__ void pull_periodic(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t z,
+                                              const uint32_t xs[3], const uint32_t ys[3], const uint32_t zs[3],
+                                              typename Codec<Tag>::C d[Q]) {
+    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
+    const uint32_t DB = p.dir_bytes;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
+        if (CZ[i] == 0)
+            d[i] = ld<Tag, true>(row, i, DB, z * ES);
+        else
+            d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Pull streaming with boundary handling.  Produces the pre-collision
 // deviations d_i(x) = f_i(x) - w_i of the reference state after streaming and
@@ -388,20 +405,33 @@ __device__ __forceinline__ void pull(const void* in, const Params<typename Codec
     const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1};
     const uint32_t DB = p.dir_bytes;
     // source index 0/1/2 along an axis for velocity component +1/0/-1
-    if constexpr (BC != BC_HALFWAY) {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
-            if (CZ[i] == 0)
-                d[i] = ld<Tag, true>(row, i, DB, z * ES);
-            else
-                d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
-        }
-    }
+    if constexpr (BC != BC_HALFWAY) pull_periodic<Tag, NT>(in, p, z, xs, ys, zs, d);
     if constexpr (BC == BC_YWALLS) ywall_swap<C>(p, y, d);
     if constexpr (BC == BC_HALFWAY) {
         const uint32_t wm = p.wall_mask;
         const char* self = row_of(in, p, x, y);
+        // A row off the x and y wall faces, in a lattice without solids, can
+        // only meet the z faces, and only in its lanes at z = 0 and z = nz - 1
+        // (the row is wavefront-uniform, so this branch is too).  Such a row
+        // takes the periodic pull and then bounces those lanes' z-crossing
+        // directions back: the same loads and values as the general path
+        // below, without its per-direction selects in every lane.
+        const bool edge_row = ((wm & 1u) && x == 0) || ((wm & 2u) && x + 1 == p.nx) ||
+                              ((wm & 4u) && y == 0) || ((wm & 8u) && y + 1 == p.ny);
+        if (!edge_row && !p.solid) {
+            pull_periodic<Tag, NT>(in, p, z, xs, ys, zs, d);
+            const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
+            if (zlo || zhi) {
+#pragma unroll
+                for (int i = 0; i < Q; ++i) {
+                    if (CZ[i] == 0 || !(CZ[i] > 0 ? zlo : zhi)) continue;
+                    const int f = CZ[i] > 0 ? 4 : 5;
+                    const C v = ld_at<Tag, NT>(self + (uint32_t)OPP[i] * DB + z * ES, i);
+                    d[i] = v + C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[f][0], p.face_u[f][1], p.face_u[f][2]);
+                }
+            }
+            return;
+        }
         // Solid source cells: all flags are loaded before any population, so
         // that no population load waits on a flag (none when there are no
         // solids: the branch is uniform).
