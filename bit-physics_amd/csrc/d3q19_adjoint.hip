@@ -614,6 +614,8 @@ int d3q19_adjoint_destroy(d3q19_adjoint* A) {
 
 int d3q19_adjoint_set_stream(d3q19_adjoint* A, void* stream) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
+    // work queued on the previous stream completes before any on the new one
+    if ((hipStream_t)stream != A->stream) D3Q19_HIPCHK(hipStreamSynchronize(A->stream));
     A->stream = (hipStream_t)stream;
     return D3Q19_OK;
 }
@@ -621,6 +623,9 @@ int d3q19_adjoint_set_stream(d3q19_adjoint* A, void* stream) {
 int d3q19_adjoint_forward(d3q19_adjoint* A, const void* u0, const void* force, const void* brinkman,
                           void* u_final, int32_t dt, int32_t on_device) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
+    // a forward pass that fails (part-way, the checkpoints then belong to no
+    // single trajectory) leaves backward refused until one completes
+    A->have_forward = false;
     if (!u0 || !force || !u_final) return set_error(D3Q19_EINVAL, "null array");
     if (A->brink && !brinkman) return set_error(D3Q19_EINVAL, "this adjoint needs a Brinkman field");
     if (dt != D3Q19_F64 && dt != D3Q19_F32) return set_error(D3Q19_EINVAL, "dtype must be F64 or F32");

@@ -711,6 +711,7 @@ int d3q19_destroy(d3q19_lattice* L) {
 
 int d3q19_set_stream(d3q19_lattice* L, void* stream) {
     if (int e = check_lat(L)) return e;
+    if ((hipStream_t)stream != L->stream) HIPCHK(hipStreamSynchronize(L->stream));
     L->stream = (hipStream_t)stream;
     return D3Q19_OK;
 }
@@ -970,13 +971,20 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
     if (int e = check_lat(L)) return e;
     if (!lo || !hi) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
-    if (!L->mr_partials) {
+    if (!L->mr_partials || !L->mr_out) {
         // one slot pair per block of the population grid (<= the partials of the diagnostics)
         const size_t nb = L->rows ? (size_t)L->cfg.nx * L->cfg.ny * ((L->cfg.nz + 63) / 64) : nblocks(L->ncell);
-        if (hipMalloc((void**)&L->mr_partials, std::max<size_t>(nb, nblocks(L->ncell)) * 2 * Q * sizeof(double)) !=
-                hipSuccess ||
-            hipMalloc((void**)&L->mr_out, 2 * Q * sizeof(double)) != hipSuccess)
+        const size_t pbytes = std::max<size_t>(nb, nblocks(L->ncell)) * 2 * Q * sizeof(double);
+        if (hipMalloc((void**)&L->mr_partials, pbytes) != hipSuccess ||
+            hipMalloc((void**)&L->mr_out, 2 * Q * sizeof(double)) != hipSuccess) {
+            // leave neither buffer behind, so that the next call allocates both
+            if (L->mr_partials) (void)hipFree(L->mr_partials);
+            if (L->mr_out) (void)hipFree(L->mr_out);
+            L->mr_partials = nullptr;
+            L->mr_out = nullptr;
             return fail(D3Q19_ENOMEM, "moment-range buffers");
+        }
+        L->bytes += pbytes + 2 * Q * sizeof(double);
     }
     if (int rc = D3Q19_DISPATCH(L, launch_moment_range, L)) return rc;
     double out[2 * Q];

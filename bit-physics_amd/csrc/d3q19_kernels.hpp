@@ -230,7 +230,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 }
 
 // Final reduction of the health partials: one 256-thread block per slot.
-static __global__ void __launch_bounds__(256)
+[[maybe_unused]] static __global__ void __launch_bounds__(256)
     k_health_final(const double* __restrict__ partials, uint32_t nblocks, double* __restrict__ out) {
     __shared__ double red[4];
     const int k = blockIdx.x;
@@ -255,7 +255,7 @@ static __global__ void __launch_bounds__(256)
 // Final sum of per-block partials: one 256-thread block per component, each
 // summing in a fixed order (strided per thread, then a fixed tree), so the
 // result is the same bit for bit on every run.
-static __global__ void __launch_bounds__(BLOCK)
+[[maybe_unused]] static __global__ void __launch_bounds__(BLOCK)
     k_sum_partials(const double* __restrict__ partials, uint32_t nblocks, int ncomp,
                    double* __restrict__ out) {
     __shared__ double red[BLOCK / 64];
@@ -275,7 +275,7 @@ static __global__ void __launch_bounds__(BLOCK)
 }
 
 // FP64 enstrophy partials 0.5 |curl u|^2 with periodic central differences.
-static __global__ void __launch_bounds__(BLOCK)
+[[maybe_unused]] static __global__ void __launch_bounds__(BLOCK)
     k_enstrophy(const float* __restrict__ u, uint32_t nx, uint32_t ny, uint32_t nz,
                 double* __restrict__ partials) {
     __shared__ double red[BLOCK / 64];

@@ -88,6 +88,9 @@ void d3q19_config_default(d3q19_config* cfg);
  * through device buffers of their size, device arrays are used in place. */
 int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out);
 int d3q19_destroy(d3q19_lattice* lat);
+/* Launch the lattice's later work on `stream` (NULL: the default stream).  Work
+   already queued on the previous stream is waited for first, so that nothing
+   queued on the new stream overtakes it. */
 int d3q19_set_stream(d3q19_lattice* lat, void* stream);
 /* Set the state to populations f (pre-collision, the reference's f^n). */
 int d3q19_set_populations(d3q19_lattice* lat, const void* f, int32_t dtype, int32_t on_device);
@@ -211,12 +214,14 @@ void d3q19_adjoint_options_default(d3q19_adjoint_options* opt);
 int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* opt, void* stream,
                          d3q19_adjoint** out);
 int d3q19_adjoint_destroy(d3q19_adjoint* adj);
+/* As d3q19_set_stream: work queued on the previous stream is waited for first. */
 int d3q19_adjoint_set_stream(d3q19_adjoint* adj, void* stream);
 /* Run the forward map from (u0, F, s), keep checkpoints, write u^T.  s is
    ignored (may be NULL) unless the options enable the Brinkman parameter. */
 int d3q19_adjoint_forward(d3q19_adjoint* adj, const void* u0, const void* force, const void* brinkman,
                           void* u_final, int32_t dtype, int32_t on_device);
-/* Given dL/du^T, write dL/du0, dL/dF and dL/ds (any may be NULL). Needs a forward first. */
+/* Given dL/du^T, write dL/du0, dL/dF and dL/ds (any may be NULL). Needs a
+   forward pass that completed: after a failed one it returns D3Q19_ESTATE. */
 int d3q19_adjoint_backward(d3q19_adjoint* adj, const void* grad_u_final, void* grad_u0,
                            void* grad_force, void* grad_brinkman, int32_t dtype, int32_t on_device);
 int32_t d3q19_adjoint_checkpoint_every(const d3q19_adjoint* adj);

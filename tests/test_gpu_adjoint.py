@@ -116,6 +116,18 @@ def test_adjoint_errors(mods):
     s = ag.DifferentiableLBM((4, 4, 4), 0.6, 5)
     with pytest.raises(D3Q19Error, match="ESTATE"):
         s.backward_raw(torch.zeros((3, 4, 4, 4), device="cuda"))
+    # a completed forward pass enables backward; a failed one disables it again
+    u0 = torch.zeros((3, 4, 4, 4), device="cuda")
+    s.forward_raw(u0, u0)
+    s.backward_raw(u0)
+    from lattice_boltzmann_d3q19_amd._device import ptr
+    from lattice_boltzmann_d3q19_amd._lib import call
+
+    with pytest.raises(D3Q19Error, match="EINVAL"):
+        call("d3q19_adjoint_forward", s._h, ptr(u0), ptr(u0), None, ptr(u0), 99, 1)
+    with pytest.raises(D3Q19Error, match="ESTATE"):
+        s.backward_raw(u0)
+    s.close()
     with pytest.raises(D3Q19Error, match="EINVAL"):
         ag.DifferentiableLBM((4, 4, 4), 0.6, 5, storage="f16")
 
