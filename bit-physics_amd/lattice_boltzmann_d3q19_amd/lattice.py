@@ -129,6 +129,14 @@ class Lattice:
     def _sync_stream(self):
         call("d3q19_set_stream", self._h, self._stream_arg())
 
+    def _copied(self, on_dev):
+        """A device input (or its converted copy) is read by work queued on the
+        lattice's stream.  On torch's current stream, torch's allocator orders
+        any reuse of the tensor's memory after that work; on a caller's own
+        stream it does not, so wait for the copy there."""
+        if on_dev and self._stream is not None:
+            call("d3q19_synchronize", self._h)
+
     def close(self):
         if getattr(self, "_h", None):
             load().d3q19_destroy(self._h)
@@ -177,6 +185,7 @@ class Lattice:
         self._sync_stream()
         p, code, on_dev, keep = _host_or_device(f, np.float64)
         call("d3q19_set_populations", self._h, p, code, on_dev)
+        self._copied(on_dev)
         return self
 
     def set_force_field(self, force):
@@ -188,7 +197,7 @@ class Lattice:
         self._sync_stream()
         p, code, on_dev, keep = _host_or_device(force, np.float64)
         call("d3q19_set_force_field", self._h, p, code, on_dev)
-        self._force_keep = keep
+        self._copied(on_dev)
         return self
 
     def set_solid(self, solid):
@@ -214,6 +223,7 @@ class Lattice:
             return self
         p, code, on_dev, keep = _host_or_device(s, np.float64)
         call("d3q19_set_brinkman", self._h, p, code, on_dev)
+        self._copied(on_dev)
         return self
 
     # ---- stepping -------------------------------------------------------
@@ -268,6 +278,10 @@ class Lattice:
         return rho, u
 
     def moments_into(self, rho=None, u=None):
+        """Write rho (Nx, Ny, Nz) and/or u (3, Nx, Ny, Nz) of f^n into device
+        tensors of one dtype (float32 or float64); either may be None."""
+        if rho is None and u is None:
+            return rho, u
         self._sync_stream()
         t = dev.torch()
         ref = rho if rho is not None else u
@@ -324,8 +338,13 @@ class Lattice:
             raw = np.ascontiguousarray(z["state"])
             if raw.nbytes != int(load().d3q19_state_bytes(self._h)):
                 raise ValueError("checkpoint state size differs from this lattice")
-            if "moment_lo" in z.files and self._me_ranges is None:
-                self.set_moment_ranges(z["moment_lo"], z["moment_hi"])
+            if "moment_lo" in z.files:
+                lo, hi = z["moment_lo"], z["moment_hi"]
+                if self._me_ranges is None:
+                    self.set_moment_ranges(lo, hi)
+                elif not (np.array_equal(lo, self._me_ranges[0]) and np.array_equal(hi, self._me_ranges[1])):
+                    # the codes only mean the saved state under the ranges they were made with
+                    raise ValueError("checkpoint moment ranges differ from this lattice's")
             self._sync_stream()
             call("d3q19_write_state", self._h, ctypes.c_void_p(raw.ctypes.data), int(z["local"]),
                  int(z["step_count"]))

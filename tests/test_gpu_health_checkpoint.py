@@ -116,3 +116,45 @@ def test_restore_rejects_other_lattice(amd, tmp_path):
         amd.Lattice((6, 5, 5), 0.7).restore(tmp_path / "ck.npz")
     with pytest.raises(ValueError):
         amd.Lattice((6, 5, 4), 0.7, storage="f16").restore(tmp_path / "ck.npz")
+    # moment codes mean the saved state only under the ranges they were made with
+    m = oracle.me_moments(f).reshape(19, -1)
+    lo, hi = m.min(1) - 0.1, m.max(1) + 0.1
+    me = amd.Lattice((6, 5, 4), 0.7, storage="me16").set_moment_ranges(lo, hi).set_populations(f)
+    me.checkpoint(tmp_path / "me.npz")
+    same = amd.Lattice((6, 5, 4), 0.7, storage="me16").set_moment_ranges(lo, hi).restore(tmp_path / "me.npz")
+    assert np.array_equal(same.populations(), me.populations())
+    with pytest.raises(ValueError, match="moment ranges"):
+        amd.Lattice((6, 5, 4), 0.7, storage="me16").set_moment_ranges(lo, hi + 0.1).restore(tmp_path / "me.npz")
+
+
+def test_lattice_on_callers_stream(amd):
+    """A lattice on a stream of the caller's own runs as one on torch's current
+    stream, bit for bit, although its device inputs (and their converted
+    copies) are freed as soon as each call returns and torch reuses the memory."""
+    import torch
+
+    print_case("callers-stream")
+    shape = (8, 6, 70)
+    f, F = seeded(shape, 9)
+
+    def inputs():
+        return (torch.tensor(f, device="cuda", dtype=torch.float32),
+                torch.tensor(F, device="cuda", dtype=torch.float32))
+
+    f32, F32 = inputs()
+    ref = amd.Lattice(shape, 0.62, storage="f16", force=F32).set_populations(f32.to(torch.float64)).step(12)
+    s = torch.cuda.Stream()
+    f32, F32 = inputs()
+    f64 = f32.to(torch.float64)
+    s.wait_stream(torch.cuda.current_stream())
+    lat = amd.Lattice(shape, 0.62, storage="f16", force=F32, stream=s.cuda_stream)
+    del F32
+    lat.set_populations(f64)
+    del f32, f64
+    # freed blocks handed out again on torch's stream and overwritten
+    junk = [torch.full((19,) + shape, float("nan"), device="cuda", dtype=torch.float64) for _ in range(2)]
+    lat.step(12)
+    del junk
+    assert np.array_equal(lat.populations(), ref.populations())
+    ref.close()
+    lat.close()
