@@ -232,7 +232,12 @@ napi_value Step(napi_env env, napi_callback_info info) {
     Wrapped* w = unwrap(env, info, &argc, argv);
     if (!w) return nullptr;
     int32_t n = 1;
-    if (argc >= 1) napi_get_value_int32(env, argv[0], &n);
+    napi_valuetype t = napi_undefined;
+    if (argc >= 1) napi_typeof(env, argv[0], &t);
+    if (t != napi_undefined && napi_get_value_int32(env, argv[0], &n) != napi_ok) {
+        napi_throw_type_error(env, nullptr, "step(n): n must be a number");
+        return nullptr;
+    }
     int st = d3q19_step(w->lat, n);
     if (st) return throw_status(env, "d3q19_step", st);
     st = d3q19_synchronize(w->lat);

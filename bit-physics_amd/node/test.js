@@ -65,6 +65,7 @@ const out = { version: version() };
   let threw = false;
   try { lat.step(-1); } catch (e) { threw = /EINVAL|status 1/.test(String(e)); }
   assert(threw, 'negative step count must throw');
+  assert.throws(() => lat.step('ten'), TypeError, 'a step count that is not a number must throw');
   lat.close();
 }
 {
