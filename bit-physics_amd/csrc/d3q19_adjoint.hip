@@ -311,6 +311,7 @@ struct d3q19_adjoint {
     void* Hfinal = nullptr;
     bool have_forward = false;
     size_t bytes = 0;
+    uint32_t lds_per_cu = 0;  // LDS bytes per CU of the device, 0 if unknown
 };
 
 namespace {
@@ -361,18 +362,22 @@ int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
     dim3 g, b;
     adj_grid<Tag>(A, g, b);
     const R* src = (const R*)in;
+    // the forward step takes the lattice's resident-block cap (the backward
+    // step's registers already hold it to 6 (FP32) and 3 (FP64) blocks per CU,
+    // and lower caps measured slower)
+    const uint32_t lds = reserve_lds(A->lds_per_cu, step_blocks_per_cu<Tag>());
     switch (A->cfg.boundary) {
         case D3Q19_BC_HALFWAY:
-            k_step<Tag, BC_HALFWAY, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_HALFWAY, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         case D3Q19_BC_YWALLS:
             if (first)
                 k_collide_local<Tag, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             else
-                k_step<Tag, BC_YWALLS, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+                k_step<Tag, BC_YWALLS, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         default:
-            k_step<Tag, BC_PERIODIC, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_PERIODIC, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
     }
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
@@ -582,6 +587,7 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     A->K = K;
     A->nck = (A->T + K - 1) / K;
     A->stream = (hipStream_t)stream;
+    A->lds_per_cu = device_lds_per_cu(cfg->device);
     const size_t vec = 3 * (size_t)ncell * A->es;
     const size_t scal = (size_t)ncell * A->es;
     bool ok = true;

@@ -89,6 +89,7 @@ struct d3q19_lattice {
     double* mr_out = nullptr;
     int64_t steps = 0;
     size_t bytes = 0;
+    uint32_t lds_per_cu = 0;  // LDS bytes per CU of the device, 0 if unknown
 };
 
 namespace {
@@ -132,13 +133,22 @@ typename Codec<Tag>::raw* buf_of(const d3q19_lattice* L, int which) {
 struct StepGrid {
     dim3 grid, block;
     uint32_t nblocks;
+    uint32_t step_lds = 0;  // dynamic LDS of a fused-step block (cap_lds)
 };
+
+// Dynamic LDS of a fused-step block: caps the resident blocks per CU at
+// step_blocks_per_cu<Tag>().
+template <class Tag>
+uint32_t cap_lds(const d3q19_lattice* L) {
+    return reserve_lds(L->lds_per_cu, step_blocks_per_cu<Tag>());
+}
 
 template <class Tag>
 StepGrid step_grid(const d3q19_lattice* L) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
     const uint32_t block = step_block<Tag>();
+    sg.step_lds = cap_lds<Tag>(L);
     if (!L->rows) {
         sg.block = dim3(block, 1, 1);
         sg.grid = dim3((L->ncell + block - 1) / block, 1, 1);
@@ -158,13 +168,13 @@ void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Ta
                    typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_step<Tag, BC_PERIODIC, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_PERIODIC, FORCED, ROWS><<<sg.grid, sg.block, sg.step_lds, L->stream>>>(in, out, p, r, v);
             break;
         case D3Q19_BC_YWALLS:
-            k_step<Tag, BC_YWALLS, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_YWALLS, FORCED, ROWS><<<sg.grid, sg.block, sg.step_lds, L->stream>>>(in, out, p, r, v);
             break;
         default:
-            k_step<Tag, BC_HALFWAY, FORCED, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
+            k_step<Tag, BC_HALFWAY, FORCED, ROWS><<<sg.grid, sg.block, sg.step_lds, L->stream>>>(in, out, p, r, v);
             break;
     }
 }
@@ -672,6 +682,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
         L->lat_bytes = (size_t)Q * ncell * es;
     }
     L->stream = (hipStream_t)stream;  // NULL: the default stream
+    L->lds_per_cu = device_lds_per_cu(cfg->device);
     const size_t lat_bytes = L->lat_bytes;
     // partial sums: one slot per block of the population grid and of the
     // linear 256-thread grids (ncell / 256)

@@ -119,11 +119,29 @@ template <> struct Codec<TBF16> {
 // Same format as TBF16; stores round stochastically (store_local).
 template <> struct Codec<TBF16S> : Codec<TBF16> {};
 
-// Threads per block of the population kernels: one 1 KiB row segment per
-// direction and block (256 cells of 4 or 8 bytes, 512 cells of 2 bytes).
+// Threads per block of the population kernels: one row segment of 256 cells
+// (4 or 8 bytes) or 512 cells (2 bytes) per direction and block.
 template <class Tag>
 constexpr int step_block() {
     return sizeof(typename Codec<Tag>::raw) == 2 ? 512 : 256;
+}
+
+// Blocks of the fused step resident per CU, 0 = as many as registers allow.
+// The step streams 19 row segments in and 19 out per block; for some storages
+// fewer concurrent blocks per CU keep HBM busier (256^3 on MI355X, GLUPS
+// uncapped -> capped): FP64 18.65 -> 19.63 at 2 blocks (uncapped: 5, by
+// registers); FP32 lid-driven cavity 39.24 -> 40.31 at 4 blocks, periodic
+// unchanged (39.9); FP16 80.5 -> 82.9 and BF16 82.4 -> 83.5 at 3 blocks of
+// 512 threads (uncapped: 4).  Stochastically rounded BF16, with more ALU work
+// per cell, runs best uncapped (82.9 against 81.1 at 3 blocks).
+template <class Tag>
+constexpr uint32_t step_blocks_per_cu() {
+    if constexpr (std::is_same<Tag, TBF16S>::value) {
+        return 0;
+    } else {
+        constexpr size_t es = sizeof(typename Codec<Tag>::raw);
+        return es == 8 ? 2 : (es == 4 ? 4 : 3);
+    }
 }
 
 // ---------------------------------------------------------------------------
