@@ -359,8 +359,9 @@ int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
     const Params<C> p = adj_params<C>(A, A->es);
-    dim3 g, b;
-    adj_grid<Tag>(A, g, b);
+    dim3 g, b, gf, bf;
+    adj_grid<Tag>(A, g, b);                       // k_collide_local
+    adj_grid<Tag>(A, gf, bf, fused_block<Tag>());  // k_step
     const R* src = (const R*)in;
     // the forward step takes the lattice's resident-block cap (the backward
     // step's registers already hold it to 6 (FP32) and 3 (FP64) blocks per CU,
@@ -368,16 +369,16 @@ int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
     const uint32_t lds = reserve_lds(A->lds_per_cu, step_blocks_per_cu<Tag>());
     switch (A->cfg.boundary) {
         case D3Q19_BC_HALFWAY:
-            k_step<Tag, BC_HALFWAY, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_HALFWAY, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         case D3Q19_BC_YWALLS:
             if (first)
                 k_collide_local<Tag, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             else
-                k_step<Tag, BC_YWALLS, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+                k_step<Tag, BC_YWALLS, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         default:
-            k_step<Tag, BC_PERIODIC, true><<<g, b, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_PERIODIC, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
     }
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;

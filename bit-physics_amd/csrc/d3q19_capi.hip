@@ -143,11 +143,11 @@ uint32_t cap_lds(const d3q19_lattice* L) {
     return reserve_lds(L->lds_per_cu, step_blocks_per_cu<Tag>());
 }
 
+// block: threads per block (step_block, or fused_block for k_step).
 template <class Tag>
-StepGrid step_grid(const d3q19_lattice* L) {
+StepGrid step_grid(const d3q19_lattice* L, uint32_t block = step_block<Tag>()) {
     StepGrid sg;
     const uint32_t nx = L->cfg.nx, ny = L->cfg.ny, nz = L->cfg.nz;
-    const uint32_t block = step_block<Tag>();
     sg.step_lds = cap_lds<Tag>(L);
     if (!L->rows) {
         sg.block = dim3(block, 1, 1);
@@ -182,7 +182,8 @@ void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Ta
 template <class Tag, bool ROWS>
 int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     typedef typename Codec<Tag>::C C;
-    const StepGrid sg = step_grid<Tag>(L);
+    const StepGrid sg = step_grid<Tag>(L);                     // k_collide_local
+    const StepGrid sf = step_grid<Tag>(L, fused_block<Tag>());  // k_step
     for (int32_t s = 0; s < n; ++s) {
         const Params<C> p = make_params<C>(L);
         const bool forced = p.force_mode != D3Q19_FORCE_NONE || p.brink != nullptr;
@@ -195,9 +196,9 @@ int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
             k_collide_local<Tag, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             L->local = false;
         } else if (forced) {
-            launch_k_step<Tag, true, ROWS>(L, sg, in, out, p, r, v);
+            launch_k_step<Tag, true, ROWS>(L, sf, in, out, p, r, v);
         } else {
-            launch_k_step<Tag, false, ROWS>(L, sg, in, out, p, r, v);
+            launch_k_step<Tag, false, ROWS>(L, sf, in, out, p, r, v);
         }
         HIPCHK(hipGetLastError());
         L->cur ^= 1;

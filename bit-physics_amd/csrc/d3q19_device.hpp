@@ -126,21 +126,33 @@ constexpr int step_block() {
     return sizeof(typename Codec<Tag>::raw) == 2 ? 512 : 256;
 }
 
-// Blocks of the fused step resident per CU, 0 = as many as registers allow.
-// The step streams 19 row segments in and 19 out per block; for some storages
-// fewer concurrent blocks per CU keep HBM busier (256^3 on MI355X, GLUPS
-// uncapped -> capped): FP64 18.65 -> 19.63 at 2 blocks (uncapped: 5, by
-// registers); FP32 lid-driven cavity 39.24 -> 40.31 at 4 blocks, periodic
-// unchanged (39.9); FP16 80.5 -> 82.9 and BF16 82.4 -> 83.5 at 3 blocks of
-// 512 threads (uncapped: 4).  Stochastically rounded BF16, with more ALU work
-// per cell, runs best uncapped (82.9 against 81.1 at 3 blocks).
+// Threads per block of the fused step (k_step).  FP32 steps two rows of 256
+// cells per block: 512 threads, 2 blocks per CU (below), ran 40.2-40.6
+// against 39.97 GLUPS for 256 threads at 4 blocks per CU, the same 16
+// wavefronts per CU (256^3 Taylor-Green; reference y-walls 40.8-40.9 against
+// 40.0-40.2, lid-driven cavity 40.2 against 40.3).  512 threads uncapped
+// ran 39.4, at 3 blocks 39.2.
+template <class Tag>
+constexpr int fused_block() {
+    return std::is_same<Tag, TF32>::value ? 512 : step_block<Tag>();
+}
+
+// Blocks of the fused step (fused_block threads) resident per CU, 0 = as
+// many as registers allow.  The step streams 19 row segments in and 19 out
+// per row; for some storages fewer concurrent blocks per CU keep HBM busier
+// (256^3 on MI355X, GLUPS uncapped -> capped): FP64 18.65 -> 19.63 at 2
+// blocks of 256 (uncapped: 5, by registers); FP32 lid-driven cavity 39.24 ->
+// 40.31 at 4 blocks of 256, periodic unchanged (39.9), then 2 blocks of 512
+// (fused_block); FP16 80.5 -> 82.9 and BF16 82.4 -> 83.5 at 3 blocks of 512
+// (uncapped: 4).  Stochastically rounded BF16, with more ALU work per cell,
+// runs best uncapped (82.9 against 81.1 at 3 blocks).
 template <class Tag>
 constexpr uint32_t step_blocks_per_cu() {
     if constexpr (std::is_same<Tag, TBF16S>::value) {
         return 0;
     } else {
         constexpr size_t es = sizeof(typename Codec<Tag>::raw);
-        return es == 8 ? 2 : (es == 4 ? 4 : 3);
+        return es == 8 ? 2 : (es == 4 ? 2 : 3);
     }
 }
 

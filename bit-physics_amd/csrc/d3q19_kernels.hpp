@@ -13,7 +13,7 @@ constexpr int BLOCK = 256;
 // (optionally) rho and u of the reference state f^n from the same pass.
 // ---------------------------------------------------------------------------
 template <class Tag, int BC, bool FORCED, bool ROWS = true>
-static __global__ void __launch_bounds__(step_block<Tag>())
+static __global__ void __launch_bounds__(fused_block<Tag>())
     k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
