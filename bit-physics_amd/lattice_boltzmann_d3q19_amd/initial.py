@@ -54,6 +54,17 @@ def random_solenoidal(shape, u_rms: float, seed: int = 42, k0: float = 8.0, dtyp
     # shell-summed energy follows E(k): per-mode |u_k|^2 ~ E(k) / k^2
     with np.errstate(divide="ignore", invalid="ignore"):
         amp = np.where(k > 0, np.sqrt(k ** 4 * np.exp(-2.0 * (k / k0) ** 2) / np.maximum(k2, 1e-30)), 0.0)
+    # Nyquist modes of even axes carry no velocity: their wavenumber has no sign
+    # (+-n/2 alias), so the inverse real transform would undo their projection.
+    # (Where the spectrum has decayed there, e.g. 384^3 with k0 = 8, they are 0.)
+    nyq = np.zeros(amp.shape, bool)
+    if nx % 2 == 0:
+        nyq |= np.abs(kx) == nx // 2
+    if ny % 2 == 0:
+        nyq |= np.abs(ky) == ny // 2
+    if nz % 2 == 0:
+        nyq |= kz == nz // 2
+    amp = np.where(nyq, 0.0, amp)
     uh = []
     for _ in range(3):
         w = rng.standard_normal(shape)
@@ -62,7 +73,7 @@ def random_solenoidal(shape, u_rms: float, seed: int = 42, k0: float = 8.0, dtyp
     with np.errstate(divide="ignore", invalid="ignore"):
         kdotu = (kx * uh[0] + ky * uh[1] + kz * uh[2]) / np.where(k2 > 0, k2, 1.0)
     uh = [uh[0] - kx * kdotu, uh[1] - ky * kdotu, uh[2] - kz * kdotu]
-    u = np.stack([np.fft.irfftn(c, s=shape) for c in uh]).astype(np.float64)
+    u = np.stack([np.fft.irfftn(c, s=shape, axes=(0, 1, 2)) for c in uh]).astype(np.float64)
     rms = np.sqrt(np.mean(np.sum(u * u, axis=0)) / 3.0)
     u *= u_rms / rms
     return u.astype(dtype)
