@@ -196,7 +196,7 @@ def study_inverse(torch, n=128, steps=2000, K=50, tau=0.6, ndir=16, eps=1e-3, se
         loss.backward()
         if s is not f64:
             s.close()
-        return float(loss), a.grad.double(), b.grad.double()
+        return float(loss.detach()), a.grad.double(), b.grad.double()
 
     L32, gu32, gF32 = grad("f32")
     L64, gu64, gF64 = grad("f64")
