@@ -292,6 +292,37 @@ def study_channel(amd, torch, shape=(256, 128, 128), tau=0.6, steps=4000, gx=1e-
     return out
 
 
+def study_cavity(amd, torch, n=256, steps=1000, warmup=20, lid=0.05):
+    """Step rates of the wall modes at n^3 (GLUPS, HIP events): the lid-driven
+    cavity (half-way walls on all six faces, lid moving +x on the top y face),
+    half-way walls on z only, the reference's y-walls with a uniform force, and
+    the periodic lattice, in FP32 and FP16 storage."""
+    shape = (n, n, n)
+    cases = {
+        "cavity": dict(boundary="halfway", wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                       face_velocity={"+y": (lid, 0.0, 0.0)}),
+        "zwalls": dict(boundary="halfway", wall_faces=("-z", "+z")),
+        "ywalls": dict(boundary="ywalls", force=(1e-6, 0.0, 0.0)),
+        "periodic": dict(),
+    }
+    out = {"study": "cavity", "workload": f"wall-modes-{n}^3-tau0.6-{steps}steps", "GLUPS": {}}
+    stream = torch.cuda.current_stream()
+    for name, kw in cases.items():
+        for storage in ("f32", "f16"):
+            lat = amd.Lattice(shape, 0.6, storage=storage, **kw).set_equilibrium()
+            lat.step(warmup)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            lat.step(steps)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            out["GLUPS"][f"{name}_{storage}"] = round(n ** 3 * steps / (e0.elapsed_time(e1) * 1e-3) / 1e9, 2)
+            lat.close()
+            print(f"[study] cavity {name} {storage}: {out['GLUPS'][f'{name}_{storage}']} GLUPS",
+                  file=sys.stderr, flush=True)
+    return out
+
+
 def study_mms(levels=(32, 64, 128, 256, 512, 1024), storages=("f64", "f32")):
     """Forced Taylor-Green MMS ladder on the fused step (solution verification):
     RMS velocity error at t_final per level and storage, observed orders."""
@@ -421,7 +452,7 @@ def main() -> int:
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-me16", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms"), default=None,
+    ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms", "cavity"), default=None,
                     help="run a validation study instead of the headline benchmark")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -447,6 +478,9 @@ def main() -> int:
         return 0
     if args.study == "mms":
         print(json.dumps(study_mms()), flush=True)
+        return 0
+    if args.study == "cavity":
+        print(json.dumps(study_cavity(amd, torch)), flush=True)
         return 0
     n = args.size
     shape = (n, n, n)
