@@ -22,6 +22,13 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
                           Couette linear profile, mass conservation
   diagnostics             brute-force NumPy sums
   inverse_grad            central finite differences of inverse_loss
+  me_basis, me_moments    M M^-1 = I, rows 0..3 the density and momentum
+                          moments (tests/test_oracle_moment_codec.py)
+  me_encode, me_decode    round trip within the closed-form bound
+                          (hi - lo) / 2 / 65535
+  run_me                  mass within the quantization budget; the published
+                          quantized-Poiseuille measurement against its FP64
+                          parent (lbm-quantized tolerance)
 """
 
 from __future__ import annotations
