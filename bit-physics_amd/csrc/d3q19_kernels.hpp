@@ -26,14 +26,7 @@ static __global__ void __launch_bounds__(fused_block<Tag>())
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-        // FP64 storage: the wavefront issues its pull loads at raised priority
-        // (s_setprio), ahead of other wavefronts' collision work (256^3:
-        // 19.97 against 19.5 GLUPS).  FP32 (37.0 against 40.0) and FP16 (77.9
-        // against 82.7) ran slower with it.
-        constexpr bool PRIO = sizeof(typename Codec<Tag>::raw) == 8;
-        if constexpr (PRIO) __builtin_amdgcn_s_setprio(3);
         pull<Tag, BC>(in, p, x, y, z, d);
-        if constexpr (PRIO) __builtin_amdgcn_s_setprio(0);
         collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
     }
     store_local<Tag>(out, p, x, y, z, d);
