@@ -9,7 +9,8 @@ in/out functions named after the reference API of
 ``packages/lattice-boltzmann-d3q19/lattice_boltzmann_d3q19/reference/__init__.py:43-71``.
 Every array is float64, C-contiguous, direction-major ``(19, Nx, Ny, Nz)``.
 
-Pinning (what checks each function; see tests/test_oracle_*.py):
+Pinning (what checks each function; see tests/test_oracle_*.py; each
+check of the additions also rejects a planted mistake, see planted()):
   feq, feq_field          golden table d3q19-equilibrium.json (abs 1e-15),
                           sum(W)==1, feq(1,0)==W, moment identities (PBT)
   density/momentum_field  closed-form moments of feq, brute-force sums
@@ -19,8 +20,15 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
   bounce_back_y_walls     reference golden vectors (tests/golden), the
                           published Poiseuille/Couette maxima
   run_halfway             Poiseuille parabola at the exact-wall tau,
-                          Couette linear profile, mass conservation
-  diagnostics             brute-force NumPy sums
+                          Couette linear profile, mass conservation; solid
+                          cells: mass conservation in a box of solid blocks,
+                          the parabola of a channel between solid rows (walls
+                          at y = 1/2 and ny - 3/2), bit-identity with face
+                          walls (tests/test_oracle_pins.py)
+  diagnostics             NumPy sums written from the definitions (mass,
+                          momentum, KE = 0.5 sum rho |u|^2 with the
+                          Guo-shifted u), with and without force and solids
+                          (tests/test_oracle_pins.py)
   inverse_grad            central finite differences of inverse_loss
   me_basis, me_moments    M M^-1 = I, rows 0..3 the density and momentum
                           moments (tests/test_oracle_moment_codec.py)
@@ -33,6 +41,7 @@ Pinning (what checks each function; see tests/test_oracle_*.py):
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import subprocess
 from pathlib import Path
@@ -58,7 +67,33 @@ def _load():
         return _lib
     if not _LIB_PATH.exists():
         build()
-    lib = ctypes.CDLL(str(_LIB_PATH))
+    _lib = _bind(ctypes.CDLL(str(_LIB_PATH)))
+    return _lib
+
+
+PLANTS = ("KE_NO_HALF", "KE_UNIT_RHO", "KE_NO_GUO", "SOLID_SAME_DIR", "SOLID_WRONG_SIDE")
+
+
+@contextlib.contextmanager
+def planted(name: str):
+    """Run the oracle's functions on a test build that carries one deliberate
+    mistake (``-DORC_PLANT_<name>``, listed in d3q19_oracle.c), so a test can
+    show that its independent check rejects that mistake.  Restores the
+    correct build on exit."""
+    global _lib
+    if name not in PLANTS:
+        raise ValueError(f"unknown planted mistake {name!r}; known: {PLANTS}")
+    target = f"_build/libd3q19_oracle_plant_{name}.so"
+    subprocess.run(["make", "-s", "-C", str(_HERE), target], check=True)
+    saved = _load()
+    _lib = _bind(ctypes.CDLL(str(_HERE / target)))
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
+def _bind(lib):
     P = ctypes.c_void_p
     i32, i64, f64, u32 = ctypes.c_int32, ctypes.c_int64, ctypes.c_double, ctypes.c_uint32
     sigs = {
@@ -89,7 +124,6 @@ def _load():
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _lib = lib
     return lib
 
 

@@ -35,6 +35,16 @@
  * faces and solid cells (lid-driven cavity, obstacles), the Brinkman
  * solid-fraction force, FP64 diagnostics (mass, momentum, kinetic energy),
  * and the FP64 discrete adjoint of the periodic Guo-forced step.
+ *
+ * Planted mistakes (test builds only).  Compiled with -DORC_PLANT_<NAME> the
+ * oracle carries one deliberate, plausible mistake, so that a test can show
+ * that its independent check rejects it (oracle/__init__.py: planted()):
+ *   KE_NO_HALF        kinetic energy without the 1/2
+ *   KE_UNIT_RHO       kinetic energy with rho taken as 1
+ *   KE_NO_GUO         kinetic energy from the unshifted velocity m / rho
+ *   SOLID_SAME_DIR    bounce from a solid source returns h_i instead of h_opp(i)
+ *   SOLID_WRONG_SIDE  the solid test probes x + c_i instead of the source x - c_i
+ * The default build defines none of them.
  */
 #include <math.h>
 #include <omp.h>
@@ -353,6 +363,26 @@ static inline int halfway_source(int64_t x, int64_t y, int64_t z, int i, int32_t
     return crossed >= 0;
 }
 
+/* The cell whose solid flag decides a pull from src (the source itself). */
+static inline int64_t solid_probe(int64_t x, int64_t y, int64_t z, int i, int32_t nx, int32_t ny,
+                                  int32_t nz, uint32_t wall_mask, int64_t src) {
+#ifdef ORC_PLANT_SOLID_WRONG_SIDE
+    int64_t probe;
+    int face;
+    halfway_source(x, y, z, OPP[i], nx, ny, nz, wall_mask, &probe, &face);
+    return probe;
+#else
+    (void)x, (void)y, (void)z, (void)i, (void)nx, (void)ny, (void)nz, (void)wall_mask;
+    return src;
+#endif
+}
+
+#ifdef ORC_PLANT_SOLID_SAME_DIR
+#define SOLID_BOUNCE_DIR(i) (i)
+#else
+#define SOLID_BOUNCE_DIR(i) OPP[i]
+#endif
+
 void orc_stream_halfway(int32_t nx, int32_t ny, int32_t nz, const double* h, double* out,
                         uint32_t wall_mask, const double* face_u, const uint8_t* solid) {
     init_constants();
@@ -374,8 +404,8 @@ void orc_stream_halfway(int32_t nx, int32_t ny, int32_t nz, const double* h, dou
                         double cdu = (double)CV[i][0] * uw[0] + (double)CV[i][1] * uw[1] +
                                      (double)CV[i][2] * uw[2];
                         out[i * ncell + c] = h[OPP[i] * ncell + c] + 2.0 * WT[i] * cdu * INV_CS2;
-                    } else if (solid && solid[src]) {
-                        out[i * ncell + c] = h[OPP[i] * ncell + c];
+                    } else if (solid && solid[solid_probe(x, y, z, i, nx, ny, nz, wall_mask, src)]) {
+                        out[i * ncell + c] = h[SOLID_BOUNCE_DIR(i) * ncell + c];
                     } else {
                         out[i * ncell + c] = h[i * ncell + src];
                     }
@@ -456,16 +486,24 @@ void orc_diagnostics(int64_t ncell, const double* f, const double* fu, const uin
         py += my;
         pz += mz;
         double ux = mx, uy = my, uz = mz;
+#ifndef ORC_PLANT_KE_NO_GUO
         if (fu) {
             ux += 0.5 * fu[0];
             uy += 0.5 * fu[1];
             uz += 0.5 * fu[2];
         }
+#endif
         double rs = rho_floor(r);
         ux /= rs;
         uy /= rs;
         uz /= rs;
+#if defined(ORC_PLANT_KE_NO_HALF)
+        ke += r * (ux * ux + uy * uy + uz * uz);
+#elif defined(ORC_PLANT_KE_UNIT_RHO)
+        ke += 0.5 * (ux * ux + uy * uy + uz * uz);
+#else
         ke += 0.5 * r * (ux * ux + uy * uy + uz * uz);
+#endif
     }
     out[0] = mass;
     out[1] = px;
