@@ -34,18 +34,65 @@ def bytes_per_cell(storage: str) -> int:
     return 2 * Q * {"f64": 8, "f32": 4, "f16": 2, "bf16": 2, "bf16sr": 2}[storage]
 
 
-def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup):
+class ShaderClock:
+    """The device's shader clock in MHz (libd3q19's d3q19_shader_clock: one
+    wavefront counts shader-clock ticks against the 100 MHz real-time counter
+    for ~20 us).  Read right before and after a timed window."""
+
+    def __init__(self, torch):
+        self.torch = torch
+        self.buf = torch.zeros(2, dtype=torch.int64, device="cuda")
+
+    def __call__(self) -> float:
+        import ctypes
+
+        from lattice_boltzmann_d3q19_amd._lib import call
+
+        mhz = ctypes.c_double()
+        call("d3q19_shader_clock", ctypes.c_void_p(self.buf.data_ptr()), ctypes.byref(mhz),
+             ctypes.c_void_p(self.torch.cuda.current_stream().cuda_stream))
+        return round(mhz.value, 1)
+
+
+def settle(torch, lat, settle_ms, chunk=10):
+    """Run steps of `lat` until settle_ms of device time have passed; returns
+    the number of steps.  At the start of a heavy launch sequence after host
+    idle time the MI355X shader clock dips (2400 -> ~1700 MHz) for some tens
+    of milliseconds and recovers; a VALU-heavier step (16-bit storage) slows
+    by up to 25% inside the dip, a memory-bound one barely moves."""
     stream = torch.cuda.current_stream()
+    n, ms = 0, 0.0
+    while ms < settle_ms:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        lat.step(chunk)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms += e0.elapsed_time(e1)
+        n += chunk
+    return n
+
+
+def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup, settle_ms=0.0, clock=None):
+    """Time `steps` steps after `warmup` untimed ones (plus, with settle_ms > 0,
+    untimed steps for that much device time before the warm-up)."""
+    stream = torch.cuda.current_stream()
+    settle_steps = 0
     if storage == "me16":
         # ranges of the moments over the whole run (warm-up + timed steps)
         from lattice_boltzmann_d3q19_amd.moment_encoded import moment_encoded_lattice
 
+        if settle_ms > 0:
+            raise ValueError("the moment-encoded leg runs without a settle phase")
         lat = moment_encoded_lattice(shape, tau, warmup + steps, rho=rho0, u=u0)
     else:
         lat = amd.Lattice(shape, tau, storage=storage)
         lat.set_equilibrium(rho0, u0)
+        if settle_ms > 0:
+            settle_steps = settle(torch, lat, settle_ms)
     lat.step(warmup)
     torch.cuda.synchronize()
+    clk0 = clock() if clock else None
     t0 = torch.cuda.Event(enable_timing=True)
     t1 = torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -54,10 +101,12 @@ def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup):
     t1.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
+    clk1 = clock() if clock else None
     ms = t0.elapsed_time(t1)
     diag = lat.diagnostics()
     lat.close()
-    return ms, wall, diag
+    info = {"settle_steps": settle_steps, "sclk_MHz_before": clk0, "sclk_MHz_after": clk1}
+    return ms, wall, diag, info
 
 
 def copy_peak_gbs(amd, torch, nbytes=2 << 30, reps=20):
@@ -358,6 +407,17 @@ def study_mms(levels=(32, 64, 128, 256, 512, 1024), storages=("f64", "f32")):
     return out
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo)."""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(shape, tau, rho0, u0, sample_steps):
     """The FP64 CPU oracle (C, OpenMP) on a bounded sample of the same workload."""
     import numpy as np
@@ -381,6 +441,9 @@ def cpu_baseline(shape, tau, rho0, u0, sample_steps):
         "value": ncell * sample_steps / dt / 1e9,
         "unit": "GLUPS",
         "cores": threads,
+        "cpu_model": cpu_model(),
+        "nproc": os.cpu_count(),
+        "affinity_cpus": len(os.sched_getaffinity(0)),
         "kind": "port",
         "sample": f"{sample_steps} steps of the same {shape[0]}x{shape[1]}x{shape[2]} Taylor-Green "
                   f"lattice, FP64 C restatement of bgk_step (collide+stream), OpenMP, {dt:.2f} s",
@@ -448,6 +511,8 @@ def main() -> int:
     ap.add_argument("--tau", type=float, default=0.55)
     ap.add_argument("--u0", type=float, default=0.04)
     ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--settle-ms", type=float, default=200.0,
+                    help="untimed device time before the FP16 leg's settled window")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-me16", action="store_true")
@@ -488,8 +553,10 @@ def main() -> int:
     rho0, u0 = taylor_green(shape, args.u0)
     nu = (args.tau - 0.5) / 3.0
 
+    clock = ShaderClock(torch)
     copy_gbs = copy_peak_gbs(amd, torch)
-    ms, wall, diag = time_lattice(amd, torch, shape, args.tau, "f32", rho0, u0, args.steps, args.warmup)
+    ms, wall, diag, info = time_lattice(amd, torch, shape, args.tau, "f32", rho0, u0, args.steps, args.warmup,
+                                        clock=clock)
     glups = ncell * args.steps / (ms * 1e-3) / 1e9
     kernel_ms = ms / args.steps
     achieved = bytes_per_cell("f32") * ncell / (kernel_ms * 1e-3) / 1e9
@@ -500,10 +567,16 @@ def main() -> int:
     ke_ratio = diag["kinetic_energy"] / ke0
     ke_lin = taylor_green_ke_decay(shape, nu, t_total)
 
-    traffic = None
+    # HBM bytes per launch from the committed rocprofv3 PMC summary (FETCH_SIZE
+    # and WRITE_SIZE passes scaled by the copy kernel); named with its source
+    traffic, traffic_source = None, None
     tf = ROOT / "profiles" / f"hbm_traffic_k_step_f32_tg{n}.json"
     if tf.exists():
-        traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        rec = json.loads(tf.read_text())
+        traffic = rec.get("bytes_per_launch")
+        traffic_source = {"file": str(tf.relative_to(ROOT)), "summary": rec.get("source"),
+                          "measured_at_commit": rec.get("measured_at_commit"),
+                          "counters": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, scaled by the copy kernel"}
 
     result = {
         "metric": "D3Q19 GLUPS at 256^3 FP32 vs FP16-store, % HBM roofline; adjoint/forward ratio",
@@ -529,6 +602,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
+            "traffic_source": traffic_source,
         },
         "copy_peak_GBs": round(copy_gbs, 1),
         "frac_of_copy_peak": round(achieved / copy_gbs, 4),
@@ -538,11 +612,21 @@ def main() -> int:
         "ke_ratio": ke_ratio,
         "ke_ratio_linear_theory": ke_lin,
         "mass_rel_drift": (diag["mass"] - float(np.sum(rho0))) / float(np.sum(rho0)),
+        "sclk_MHz": [info["sclk_MHz_before"], info["sclk_MHz_after"]],
     }
 
     if not args.no_fp16:
-        ms16, _, diag16 = time_lattice(amd, torch, shape, args.tau, "f16", rho0, u0, args.steps, args.warmup)
+        # Two FP16 runs of K steps after W warm-up steps: the first exactly as the
+        # FP32 leg (its kinetic energy is compared with FP32's at the same step),
+        # the second after `settle_ms` of untimed steps, past the shader-clock dip
+        # that follows the host-side set-up (the 16-bit step is VALU-heavier and
+        # slows inside the dip; the rate reported is the settled one).
+        ms16f, _, diag16, info16f = time_lattice(amd, torch, shape, args.tau, "f16", rho0, u0, args.steps,
+                                                 args.warmup, clock=clock)
+        ms16, _, _, info16 = time_lattice(amd, torch, shape, args.tau, "f16", rho0, u0, args.steps, args.warmup,
+                                          settle_ms=args.settle_ms, clock=clock)
         glups16 = ncell * args.steps / (ms16 * 1e-3) / 1e9
+        glups16f = ncell * args.steps / (ms16f * 1e-3) / 1e9
         ach16 = bytes_per_cell("f16") * ncell / (ms16 / args.steps * 1e-3) / 1e9
         result["fp16_store"] = {
             "value": round(glups16, 4),
@@ -550,11 +634,17 @@ def main() -> int:
             "ratio_vs_fp32": round(glups16 / glups, 4),
             "achieved_GBs": round(ach16, 1),
             "frac": round(ach16 / HBM_PEAK_GBS, 4),
+            "settle_ms": args.settle_ms,
+            "settle_steps": info16["settle_steps"],
+            "sclk_MHz": [info16["sclk_MHz_before"], info16["sclk_MHz_after"]],
+            "first_window": {"value": round(glups16f, 4), "ratio_vs_fp32": round(glups16f / glups, 4),
+                             "sclk_MHz": [info16f["sclk_MHz_before"], info16f["sclk_MHz_after"]],
+                             "note": "K steps after only W warm-up steps, as the FP32 leg"},
             "ke_rel_diff_vs_fp32": abs(diag16["kinetic_energy"] - diag["kinetic_energy"]) / diag["kinetic_energy"],
         }
 
     if not args.no_me16:
-        ms_me, _, diag_me = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps, args.warmup)
+        ms_me, _, diag_me, _ = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps, args.warmup)
         glups_me = ncell * args.steps / (ms_me * 1e-3) / 1e9
         ach_me = bytes_per_cell("me16") * ncell / (ms_me / args.steps * 1e-3) / 1e9
         result["me16_store"] = {

@@ -660,9 +660,15 @@ int d3q19_adjoint_forward(d3q19_adjoint* A, const void* u0, const void* force, c
 int d3q19_adjoint_backward(d3q19_adjoint* A, const void* grad_u_final, void* grad_u0, void* grad_force,
                            void* grad_brinkman, int32_t dt, int32_t on_device) {
     if (!A) return set_error(D3Q19_EINVAL, "null adjoint handle");
-    if (!A->have_forward) return set_error(D3Q19_ESTATE, "adjoint_backward needs adjoint_forward first");
+    if (!A->have_forward)
+        return set_error(D3Q19_ESTATE, "adjoint_backward needs an adjoint_forward since the last backward "
+                                       "(one backward per forward)");
     if (!grad_u_final) return set_error(D3Q19_EINVAL, "null dL/du_final");
     if (grad_brinkman && !A->brink) return set_error(D3Q19_EINVAL, "this adjoint has no Brinkman parameter");
+    // The recomputation overwrites the segment buffers, one of which holds
+    // H_T (the seed's input): a second backward would start from the wrong
+    // state, so it is refused until the next forward.
+    A->have_forward = false;
     const size_t n3 = 3 * (size_t)A->ncell;
     if (int e = stage_in(grad_u_final, dt, on_device, n3, A->gu, ct_double(A), A->stream)) return e;
     int rc = ct_double(A) ? run_backward<TF64>(A) : run_backward<TF32>(A);

@@ -78,7 +78,7 @@ struct d3q19_lattice {
     uint8_t* solid = nullptr;
     double* partials = nullptr;
     double* dout = nullptr;  // device results of reductions
-    float* uscratch = nullptr;
+    void* uscratch = nullptr;  // read-back velocity (3, ncell) in the compute type (enstrophy)
     bool has_state = false;
     bool local = false;  // buffer cur holds pre-collision f^n in place
     // D3Q19_ME16: buf[0] holds the moment codes, buf[1] the FP32 work lattice
@@ -935,15 +935,29 @@ int d3q19_enstrophy(d3q19_lattice* L, double* out) {
     if (int e = check_lat(L)) return e;
     if (!out) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    // Central differences wrap around every axis: only a fully periodic
+    // lattice (no wall face, no solid cell) has a well-defined value here.
+    const bool periodic = L->cfg.boundary == D3Q19_BC_PERIODIC ||
+                          (L->cfg.boundary == D3Q19_BC_HALFWAY && L->cfg.wall_faces == 0 && !L->solid);
+    if (!periodic)
+        return fail(D3Q19_EINVAL, "enstrophy uses periodic central differences: the lattice has walls or solids");
+    const size_t ubytes = 3 * (size_t)L->ncell * ct_size(L);
     if (!L->uscratch) {
-        if (hipMalloc((void**)&L->uscratch, 3 * (size_t)L->ncell * sizeof(float)) != hipSuccess)
-            return fail(D3Q19_ENOMEM, "velocity scratch allocation failed");
-        L->bytes += 3 * (size_t)L->ncell * sizeof(float);
+        if (hipMalloc(&L->uscratch, ubytes) != hipSuccess) return fail(D3Q19_ENOMEM, "velocity scratch allocation failed");
+        L->bytes += ubytes;
     }
-    int rc = D3Q19_DISPATCH(L, launch_readback, L, (float*)nullptr, (float*)nullptr, L->uscratch);
-    if (rc) return rc;
     const uint32_t g = nblocks(L->ncell);
-    k_enstrophy<<<g, BLOCK, 0, L->stream>>>(L->uscratch, L->cfg.nx, L->cfg.ny, L->cfg.nz, L->partials);
+    if (is_ct_double(L)) {
+        int rc = D3Q19_DISPATCH(L, launch_readback, L, (double*)nullptr, (double*)nullptr, (double*)L->uscratch);
+        if (rc) return rc;
+        k_enstrophy<double><<<g, BLOCK, 0, L->stream>>>((const double*)L->uscratch, L->cfg.nx, L->cfg.ny,
+                                                        L->cfg.nz, L->partials);
+    } else {
+        int rc = D3Q19_DISPATCH(L, launch_readback, L, (float*)nullptr, (float*)nullptr, (float*)L->uscratch);
+        if (rc) return rc;
+        k_enstrophy<float><<<g, BLOCK, 0, L->stream>>>((const float*)L->uscratch, L->cfg.nx, L->cfg.ny,
+                                                       L->cfg.nz, L->partials);
+    }
     HIPCHK(hipGetLastError());
     k_sum_partials<<<1, BLOCK, 0, L->stream>>>(L->partials, g, 1, L->dout);
     HIPCHK(hipGetLastError());
@@ -1135,6 +1149,19 @@ int d3q19_feq_field(const void* rho, const void* u, void* out, int64_t nc, int32
                                                         (uint32_t)nc);
     else
         k_fn_feq<float><<<nblocks(nc), BLOCK, 0, s>>>((const float*)rho, (const float*)u, (float*)out,
+                                                       (uint32_t)nc);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int d3q19_feq(const void* rho, const void* u, void* out, int64_t nc, int32_t dt, void* stream) {
+    if (int e = check_fn(u, out, nc, dt)) return e;
+    hipStream_t s = (hipStream_t)stream;
+    if (dt == D3Q19_F64)
+        k_fn_feq<double, true><<<nblocks(nc), BLOCK, 0, s>>>((const double*)rho, (const double*)u, (double*)out,
+                                                        (uint32_t)nc);
+    else
+        k_fn_feq<float, true><<<nblocks(nc), BLOCK, 0, s>>>((const float*)rho, (const float*)u, (float*)out,
                                                        (uint32_t)nc);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;

@@ -274,9 +274,11 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     }
 }
 
-// FP64 enstrophy partials 0.5 |curl u|^2 with periodic central differences.
+// FP64 enstrophy partials 0.5 |curl u|^2 with periodic central differences,
+// from the read-back velocity u (3, ncell) in the lattice's compute type T.
+template <class T>
 [[maybe_unused]] static __global__ void __launch_bounds__(BLOCK)
-    k_enstrophy(const float* __restrict__ u, uint32_t nx, uint32_t ny, uint32_t nz,
+    k_enstrophy(const T* __restrict__ u, uint32_t nx, uint32_t ny, uint32_t nz,
                 double* __restrict__ partials) {
     __shared__ double red[BLOCK / 64];
     const uint32_t ncell = nx * ny * nz;
@@ -289,10 +291,10 @@ static __global__ void __launch_bounds__(step_block<Tag>())
         const uint32_t xm = (x == 0 ? nx - 1 : x - 1), xp = (x + 1 == nx ? 0 : x + 1);
         const uint32_t ym = (y == 0 ? ny - 1 : y - 1), yp = (y + 1 == ny ? 0 : y + 1);
         const uint32_t zm = (z == 0 ? nz - 1 : z - 1), zp = (z + 1 == nz ? 0 : z + 1);
-        const float* ux = u;
-        const float* uy = u + ncell;
-        const float* uz = u + 2 * (size_t)ncell;
-        auto at = [&](const float* a, uint32_t i, uint32_t j, uint32_t k) {
+        const T* ux = u;
+        const T* uy = u + ncell;
+        const T* uz = u + 2 * (size_t)ncell;
+        auto at = [&](const T* a, uint32_t i, uint32_t j, uint32_t k) {
             return (double)a[i * nyz + j * nz + k];
         };
         const double dwy_dz = 0.5 * (at(uy, x, y, zp) - at(uy, x, y, zm));
@@ -363,6 +365,18 @@ __device__ __forceinline__ T ref_feq(int i, T r, T ux, T uy, T uz, T u_sq, const
 #pragma clang fp contract(off)
     T cu = T(CX[i]) * ux + T(CY[i]) * uy + T(CZ[i]) * uz;
     return wgt<T>(i) * r * (T(1) + cu * k.inv_cs2 + cu * cu * k.inv_two_cs4 - u_sq * k.inv_two_cs2);
+}
+
+// reference/equilibrium.py:34-53 (the point function feq), one direction: the
+// same polynomial as ref_feq with the reference's divisions by c_s^2,
+// 2 c_s^4 and 2 c_s^2 instead of reciprocal multiplies; the two forms differ
+// in the last bit for about a quarter of random (rho, u).
+template <class T>
+__device__ __forceinline__ T ref_feq_point(int i, T r, T ux, T uy, T uz, T u_sq) {
+#pragma clang fp contract(off)
+    const T cs2 = T(1) / T(3);
+    T cu = T(CX[i]) * ux + T(CY[i]) * uy + T(CZ[i]) * uz;
+    return wgt<T>(i) * r * (T(1) + cu / cs2 + (cu * cu) / (T(2) * cs2 * cs2) - u_sq / (T(2) * cs2));
 }
 
 // reference/bgk.py:72-110 collision of one cell.
@@ -482,7 +496,9 @@ static __global__ void __launch_bounds__(BLOCK)
 }
 
 // feq_field (reference/equilibrium.py:71-98); rho == nullptr means rho = 1.
-template <class T>
+// POINT: the point function's operation order (equilibrium.py:34-53) at each
+// of the ncell (rho, u) pairs.
+template <class T, bool POINT = false>
 static __global__ void __launch_bounds__(BLOCK)
     k_fn_feq(const T* __restrict__ rho, const T* __restrict__ u, T* __restrict__ out, uint32_t ncell) {
 #pragma clang fp contract(off)
@@ -493,7 +509,9 @@ static __global__ void __launch_bounds__(BLOCK)
     const T u_sq = ux * ux + uy * uy + uz * uz;
     const T r = rho ? rho[cell] : T(1);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) out[(size_t)i * ncell + cell] = ref_feq(i, r, ux, uy, uz, u_sq, k);
+    for (int i = 0; i < Q; ++i)
+        out[(size_t)i * ncell + cell] = POINT ? ref_feq_point(i, r, ux, uy, uz, u_sq)
+                                              : ref_feq(i, r, ux, uy, uz, u_sq, k);
 }
 
 // set_equilibrium in one pass: the state k_fn_feq followed by k_load would

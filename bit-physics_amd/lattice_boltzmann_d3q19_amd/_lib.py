@@ -38,6 +38,7 @@ EXPORTED = (
     "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
     "d3q19_moment_basis", "d3q19_set_moment_ranges", "d3q19_moment_range", "d3q19_health",
     "d3q19_set_step_count", "d3q19_state_bytes", "d3q19_read_state", "d3q19_write_state",
+    "d3q19_feq", "d3q19_shader_clock",
 )
 
 
@@ -128,6 +129,8 @@ def load() -> ctypes.CDLL:
         "d3q19_stream": (ctypes.c_int, [P, P, i32, i32, i32, i32, P]),
         "d3q19_bounce_back_y_walls": (ctypes.c_int, [P, P, P, P, i32, i32, i32, i32, P]),
         "d3q19_feq_field": (ctypes.c_int, [P, P, P, i64, i32, P]),
+        "d3q19_feq": (ctypes.c_int, [P, P, P, i64, i32, P]),
+        "d3q19_shader_clock": (ctypes.c_int, [P, ctypes.POINTER(ctypes.c_double), P]),
         "d3q19_density_field": (ctypes.c_int, [P, P, i64, i32, P]),
         "d3q19_momentum_field": (ctypes.c_int, [P, P, i64, i32, P]),
         "d3q19_macroscopic_velocity": (ctypes.c_int, [P, P, P, i64, i32, P]),

@@ -54,7 +54,9 @@ class DifferentiableLBM:
         fails if no checkpoint interval fits.
 
     Holds the checkpoints of the most recent forward call: one backward per
-    forward, in order.
+    forward, in order.  A second backward of the same forward, or the
+    backward of a forward that a later one replaced, raises instead of
+    returning gradients of the wrong trajectory.
     """
 
     def __init__(self, shape, tau, steps, checkpoint_every=0, storage="f32", boundary="periodic",
@@ -99,6 +101,7 @@ class DifferentiableLBM:
              ctypes.byref(h))
         self._h = h
         self.tdtype = dev.torch().float64 if storage == "f64" else dev.torch().float32
+        self.forward_count = 0  # forwards run; the checkpoints belong to the last one
 
     @property
     def checkpoint_every(self) -> int:
@@ -144,6 +147,7 @@ class DifferentiableLBM:
             sc = s.detach().to(self.tdtype).contiguous()
         out = t.empty_like(u0c)
         call("d3q19_adjoint_set_stream", self._h, dev.stream_ptr(self.device))
+        self.forward_count += 1
         call("d3q19_adjoint_forward", self._h, dev.ptr(u0c), dev.ptr(Fc), dev.ptr(sc), dev.ptr(out),
              self._code(), 1)
         return out
@@ -174,10 +178,16 @@ def _function_class():
         def forward(ctx, u0, force, s, solver):
             ctx.solver = solver
             ctx.in_dtypes = (u0.dtype, force.dtype, None if s is None else s.dtype)
-            return solver.forward_raw(u0, force, s)
+            out = solver.forward_raw(u0, force, s)
+            ctx.forward_id = solver.forward_count
+            return out
 
         @staticmethod
         def backward(ctx, grad_u):
+            if ctx.forward_id != ctx.solver.forward_count:
+                raise RuntimeError("DifferentiableLBM holds the checkpoints of its latest forward only: "
+                                   "this output's forward was replaced by a later call (one backward per "
+                                   "forward, in order)")
             gu0, gF, gs = ctx.solver.backward_raw(grad_u)
             d0, d1, d2 = ctx.in_dtypes
             return gu0.to(d0), gF.to(d1), (None if gs is None else gs.to(d2)), None

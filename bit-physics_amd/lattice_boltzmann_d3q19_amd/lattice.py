@@ -129,6 +129,23 @@ class Lattice:
     def _sync_stream(self):
         call("d3q19_set_stream", self._h, self._stream_arg())
 
+    def _check_out(self, t, name, shape, dtypes):
+        """Validate a device tensor that a kernel will write through its raw
+        pointer: a CUDA tensor on the lattice's device, contiguous, of exactly
+        `shape` and one of `dtypes`.  Raises ValueError before any launch."""
+        if t is None:
+            return
+        if not dev.is_tensor(t):
+            raise ValueError(f"{name} must be a torch device tensor, got {type(t).__name__}")
+        if tuple(t.shape) != tuple(shape):
+            raise ValueError(f"{name} must have shape {tuple(shape)}, got {tuple(t.shape)}")
+        if t.dtype not in dtypes:
+            raise ValueError(f"{name} must have dtype in {[str(d) for d in dtypes]}, got {t.dtype}")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+        if not t.is_cuda or t.device.index != self.device:
+            raise ValueError(f"{name} must live on cuda:{self.device}, got {t.device}")
+
     def _copied(self, on_dev):
         """A device input (or its converted copy) is read by work queued on the
         lattice's stream.  On torch's current stream, torch's allocator orders
@@ -235,6 +252,10 @@ class Lattice:
     def step_moments(self, n: int, rho_out=None, u_out=None):
         """Advance n steps; the last step's pass also writes rho/u (FP32 device
         tensors) of the state that step started from."""
+        if rho_out is not None or u_out is not None:
+            t = dev.torch()
+            self._check_out(rho_out, "rho_out", self.shape, (t.float32,))
+            self._check_out(u_out, "u_out", (3,) + self.shape, (t.float32,))
         self._sync_stream()
         call("d3q19_step_moments", self._h, int(n), dev.ptr(rho_out), dev.ptr(u_out))
         return self
@@ -261,8 +282,11 @@ class Lattice:
 
     def populations_into(self, out):
         """Write f^n into a (19, Nx, Ny, Nz) float32/float64 device tensor."""
-        self._sync_stream()
         t = dev.torch()
+        if out is None:
+            raise ValueError("out must be a (19, Nx, Ny, Nz) device tensor")
+        self._check_out(out, "out", (19,) + self.shape, (t.float32, t.float64))
+        self._sync_stream()
         code = D3Q19_F64 if out.dtype == t.float64 else D3Q19_F32
         call("d3q19_read_populations", self._h, dev.ptr(out), code, 1)
         return out
@@ -282,9 +306,13 @@ class Lattice:
         tensors of one dtype (float32 or float64); either may be None."""
         if rho is None and u is None:
             return rho, u
-        self._sync_stream()
         t = dev.torch()
         ref = rho if rho is not None else u
+        if getattr(ref, "dtype", None) not in (t.float32, t.float64):
+            raise ValueError(f"rho/u must be float32 or float64 device tensors, got {getattr(ref, 'dtype', ref)}")
+        self._check_out(rho, "rho", self.shape, (ref.dtype,))
+        self._check_out(u, "u", (3,) + self.shape, (ref.dtype,))
+        self._sync_stream()
         code = D3Q19_F64 if ref.dtype == t.float64 else D3Q19_F32
         call("d3q19_read_moments", self._h, dev.ptr(rho), dev.ptr(u), code, 1)
         return rho, u

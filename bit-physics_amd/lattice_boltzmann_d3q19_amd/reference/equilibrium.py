@@ -21,10 +21,14 @@ def feq_field(rho, u, *, precision: str = "fp64"):
 
 
 def feq(rho: float, u, *, precision: str = "fp64") -> list[float]:
-    """equilibrium.py:34-53: the 19 f_i^eq at one (rho, u)."""
-    r = np.full((1, 1, 1), float(rho))
-    uu = np.asarray([float(u[0]), float(u[1]), float(u[2])], dtype=np.float64).reshape(3, 1, 1, 1)
-    return [float(v) for v in feq_field(r, uu, precision=precision).reshape(19)]
+    """equilibrium.py:34-53: the 19 f_i^eq at one (rho, u), in the point
+    function's operation order (divisions by c_s^2, 2 c_s^4, 2 c_s^2)."""
+    code, tdt = _dt(precision)
+    r = dev.to_device(np.full(1, float(rho)), tdt)
+    uu = dev.to_device(np.asarray([float(u[0]), float(u[1]), float(u[2])], dtype=np.float64).reshape(3, 1), tdt)
+    out = dev.empty((19, 1), tdt)
+    call("d3q19_feq", dev.ptr(r), dev.ptr(uu), dev.ptr(out), 1, code, dev.stream_ptr())
+    return [float(v) for v in out.to(dtype=dev.torch().float64).cpu().numpy().reshape(19)]
 
 
 def density_field(f, *, precision: str = "fp64"):

@@ -123,7 +123,12 @@ int d3q19_diagnostics(d3q19_lattice* lat, double out[5]);
  * out = [cells with a NaN/Inf population, rho or u; min rho; max rho;
  * max |u|] over the finite cells (solid cells excluded). */
 int d3q19_health(d3q19_lattice* lat, double out[4]);
-/* FP64 enstrophy 0.5 sum |curl u|^2 (periodic central differences) of the read-back u. */
+/* FP64 enstrophy 0.5 sum |curl u|^2 of the read-back u (Guo-corrected, in the
+ * compute type: FP64 for F64 storage, else FP32), with periodic central
+ * differences 0.5 (u(x+1) - u(x-1)).  Defined for periodic lattices only:
+ * returns D3Q19_EINVAL for reference y-walls and for half-way lattices with a
+ * wall face or solid cells.  (No reference counterpart: a diagnostic of the
+ * decaying-turbulence workload.) */
 int d3q19_enstrophy(d3q19_lattice* lat, double* out);
 int d3q19_synchronize(d3q19_lattice* lat);
 int64_t d3q19_step_count(const d3q19_lattice* lat);
@@ -178,6 +183,12 @@ int d3q19_bounce_back_y_walls(const void* f, void* out, const double wall_veloci
 /* feq_field (NP/reference/equilibrium.py:71-98). */
 int d3q19_feq_field(const void* rho, const void* u, void* out, int64_t ncell, int32_t dtype,
                     void* stream);
+/* feq (NP/reference/equilibrium.py:34-53), the point function, at each of
+ * ncell (rho, u) pairs: rho (ncell) or NULL for rho = 1, u (3, ncell), out
+ * (19, ncell).  It divides by c_s^2, 2 c_s^4 and 2 c_s^2 as the point function
+ * does, where feq_field multiplies by their reciprocals, so in FP64 each
+ * value equals the reference's feq(rho, u) to the bit. */
+int d3q19_feq(const void* rho, const void* u, void* out, int64_t ncell, int32_t dtype, void* stream);
 /* density_field (NP/reference/equilibrium.py:101-103). */
 int d3q19_density_field(const void* f, void* rho, int64_t ncell, int32_t dtype, void* stream);
 /* momentum_field (NP/reference/equilibrium.py:106-113). */
@@ -232,6 +243,12 @@ size_t d3q19_adjoint_memory_bytes(const d3q19_adjoint* adj);
  * loads/stores, stream-ordered: the measured HBM copy peak and the known
  * byte count that scales rocprof FETCH_SIZE / WRITE_SIZE. */
 int d3q19_copy_kernel(const void* src, void* dst, size_t bytes, void* stream);
+/* Current shader clock in MHz: one wavefront counts shader-clock ticks
+ * against the device's constant real-time counter for about 20 us.  `ticks`
+ * is a caller-owned device buffer of 16 bytes; the call synchronises the
+ * stream.  Read between timed launches to see power-management clock
+ * transients. */
+int d3q19_shader_clock(void* ticks, double* mhz, void* stream);
 
 #ifdef __cplusplus
 }

@@ -140,3 +140,32 @@ def test_dropin_surface_matches_reference():
     assert np.array_equal(R.C[R.OPP], -R.C)
     assert np.allclose(np.einsum("i,ia,ib->ab", R.W, R.C, R.C), np.eye(3) / 3, rtol=0, atol=1e-15)
     assert R.CANONICAL_DESCRIPTOR_POISEUILLE == "poiseuille-64x32-seed42-step1000"
+
+
+def test_output_tensor_checks_reject_bad_tensors_without_gpu():
+    """Lattice's device-output paths (populations_into, moments_into,
+    step_moments) hand raw pointers to kernels that write through them: the
+    tensor's shape, dtype, contiguity and device are checked first, and a bad
+    one raises ValueError before any launch."""
+    import types
+
+    import numpy as np
+    import torch
+
+    from lattice_boltzmann_d3q19_amd.lattice import Lattice
+
+    fake = types.SimpleNamespace(device=0)
+    check = Lattice._check_out.__get__(fake)
+    shape = (19, 4, 3, 2)
+    ok_dtypes = (torch.float32, torch.float64)
+    with pytest.raises(ValueError, match="shape"):
+        check(torch.empty((19, 4, 3, 3)), "out", shape, ok_dtypes)
+    with pytest.raises(ValueError, match="dtype"):
+        check(torch.empty(shape, dtype=torch.float16), "out", shape, ok_dtypes)
+    with pytest.raises(ValueError, match="contiguous"):
+        check(torch.empty((19, 2, 3, 4)).transpose(1, 3), "out", shape, ok_dtypes)
+    with pytest.raises(ValueError, match="cuda:0"):
+        check(torch.empty(shape), "out", shape, ok_dtypes)  # a host tensor
+    with pytest.raises(ValueError, match="torch device tensor"):
+        check(np.empty(shape), "out", shape, ok_dtypes)
+    check(None, "out", shape, ok_dtypes)  # absent outputs are allowed

@@ -132,6 +132,41 @@ def test_adjoint_errors(mods):
         ag.DifferentiableLBM((4, 4, 4), 0.6, 5, storage="f16")
 
 
+def test_one_backward_per_forward(mods):
+    """The recomputation in backward overwrites the buffer that holds H_T, so
+    a second backward of one forward is refused (ESTATE), and so is the
+    backward of a forward that a later forward replaced; the next forward
+    gives the same gradients bit for bit (T = 10, K = 5)."""
+    torch, ag = mods
+    from lattice_boltzmann_d3q19_amd import D3Q19Error
+
+    print_case("adjoint-one-backward-per-forward", T=10, K=5)
+    shape = (6, 5, 4)
+    u0, F, ut = problem(shape, seed=11)
+    s = ag.DifferentiableLBM(shape, 0.6, 10, checkpoint_every=5, storage="f64")
+    u0t, Ft, g = (torch.tensor(a, device="cuda") for a in (u0, F, ut))
+    s.forward_raw(u0t, Ft)
+    first = [t.clone() for t in s.backward_raw(g)[:2]]
+    with pytest.raises(D3Q19Error, match="ESTATE"):
+        s.backward_raw(g)
+    s.forward_raw(u0t, Ft)
+    again = s.backward_raw(g)[:2]
+    assert all(torch.equal(a, b) for a, b in zip(first, again))
+    # through autograd: a retained graph cannot run the adjoint twice
+    a = u0t.clone().requires_grad_(True)
+    loss = ag.outlet_velocity(s(a, Ft))
+    loss.backward(retain_graph=True)
+    with pytest.raises(RuntimeError):
+        loss.backward()
+    # the backward of a replaced forward raises instead of using the newer checkpoints
+    a = u0t.clone().requires_grad_(True)
+    old = ag.velocity_matching_loss(s(a, Ft), g)
+    s(a.detach() * 2, Ft)
+    with pytest.raises(RuntimeError, match="replaced"):
+        old.backward()
+    s.close()
+
+
 def channel_problem(shape, seed=5):
     rng = np.random.default_rng(seed)
     u0 = 0.01 * rng.standard_normal((3,) + shape)

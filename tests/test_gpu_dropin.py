@@ -50,6 +50,39 @@ def test_fp64_dropins_bit_exact(R, shape):
                           oracle.apply_bounce_back_y_walls(f, top, bot))
 
 
+def test_point_feq_bit_exact(R):
+    """feq (equilibrium.py:34-53) divides where feq_field multiplies by
+    reciprocals: the point drop-in must follow the division form to the bit,
+    at 1,000 seeded (rho, u), the golden table's points and u = 0."""
+    import json
+
+    import torch
+
+    from lattice_boltzmann_d3q19_amd import _device as dev
+    from lattice_boltzmann_d3q19_amd._lib import D3Q19_F64, call
+
+    print_case("dropin-point-feq", points=1000)
+    rng = np.random.default_rng(7)
+    n = 1000
+    rho = 1.0 + 0.1 * rng.standard_normal(n)
+    u = 0.1 * rng.standard_normal((3, n))
+    want = np.array([oracle.feq(rho[k], u[:, k]) for k in range(n)]).T
+    r = torch.tensor(rho, device="cuda")
+    uu = torch.tensor(u, device="cuda")
+    out = torch.empty((19, n), dtype=torch.float64, device="cuda")
+    call("d3q19_feq", dev.ptr(r), dev.ptr(uu), dev.ptr(out), n, D3Q19_F64, dev.stream_ptr())
+    assert np.array_equal(out.cpu().numpy(), want)
+    # the two operation orders do differ on these inputs, so the check can tell them apart
+    field = oracle.feq_field(rho.reshape(n, 1, 1), u.reshape(3, n, 1, 1)).reshape(19, n)
+    assert not np.array_equal(field, want)
+    for k in range(25):
+        assert R.feq(rho[k], u[:, k]) == oracle.feq(rho[k], u[:, k]), k
+    for tp in json.loads((GOLDEN / "d3q19-equilibrium.json").read_text())["test_points"]:
+        got = R.feq(tp["inputs"]["rho"], tp["inputs"]["u"])
+        assert got == oracle.feq(tp["inputs"]["rho"], tp["inputs"]["u"]) == tp["expected"]["f_eq"]
+    assert R.feq(1.0, (0.0, 0.0, 0.0)) == list(R.WEIGHTS)
+
+
 def test_fp64_dropins_reproduce_reference_golden_vectors(R):
     print_case("dropin-golden")
     d = np.load(GOLDEN / "ywalls_moving.npz", allow_pickle=False)

@@ -6,9 +6,11 @@ segments of 128, a launch geometry the 256^3 workload does not use.
   below 1e-12 in f - w; FP32 storage below 1e-6 relative in populations.
 * Translation by (101, 7, 250) cells commutes with 10 steps, bit for bit, in
   FP32, FP16 and BF16 storage.
-* The whole 4000-step run against FP64 storage: kinetic energy every 500
-  steps within 1e-5 (FP32) and the 16-bit target 1e-3 (FP16, stochastically
-  rounded BF16).
+* The whole 4000-step run against FP64 storage, every 500 steps, the three
+  quantities the workload compares: kinetic energy within 1e-5 (FP32) and the
+  16-bit target 1e-3 (FP16, stochastically rounded BF16); enstrophy within
+  3e-5 (FP32) and 2e-3 (16-bit); total mass within 1e-9 (FP32) and 2e-6
+  (16-bit).
 """
 
 from __future__ import annotations
@@ -74,30 +76,41 @@ def test_translation_equivariance_bitwise(mods, storage):
     torch.cuda.empty_cache()
 
 
-def ke_series(amd, storage, u0, steps=4000, every=500):
+def series(amd, storage, u0, steps=4000, every=500):
+    """(kinetic energy, enstrophy, mass) every `every` steps, FP64 reductions."""
     lat = amd.Lattice(SHAPE, TAU, storage=storage).set_equilibrium(None, u0)
-    ke = [lat.diagnostics()["kinetic_energy"]]
-    for _ in range(steps // every):
-        ke.append(lat.step(every).diagnostics()["kinetic_energy"])
+    out = []
+    for k in range(steps // every + 1):
+        if k:
+            lat.step(every)
+        d = lat.diagnostics()
+        out.append((d["kinetic_energy"], lat.enstrophy(), d["mass"]))
     lat.close()
-    return np.array(ke)
+    return np.array(out)
 
 
 @pytest.fixture(scope="module")
-def ke_fp64(mods):
+def fp64_series(mods):
     _, amd, u0 = mods
-    return ke_series(amd, "f64", u0)
+    return series(amd, "f64", u0)
 
 
-@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("f16", 1e-3), ("bf16sr", 1e-3)])
-def test_full_run_kinetic_energy_against_fp64(mods, ke_fp64, storage, tol):
-    """The whole 4000-step run: kinetic energy every 500 steps against FP64
-    storage.  The FP16 bound is the path's 16-bit target (1e-3 relative);
-    measured FP32 5.8e-6, FP16 1.4e-4, BF16 with stochastic rounding 1.3e-4."""
+@pytest.mark.parametrize("storage,tol", [("f32", (1e-5, 3e-5, 1e-9)), ("f16", (1e-3, 2e-3, 2e-6)),
+                                         ("bf16sr", (1e-3, 2e-3, 2e-6))])
+def test_full_run_against_fp64(mods, fp64_series, storage, tol):
+    """The whole 4000-step run against FP64 storage: kinetic energy, enstrophy
+    and total mass every 500 steps.  The 16-bit KE bound is the path's target
+    (1e-3 relative).  Measured (KE / enstrophy / mass): FP32 5.8e-6 / 7.1e-6 /
+    1.8e-10, FP16 1.4e-4 / 1.7e-4 / 6.0e-8, BF16 with stochastic rounding
+    1.3e-4 / 8.8e-4 / 3.5e-7."""
     torch, amd, u0 = mods
-    print_case("full-size-384-ke", shape=SHAPE, storage=storage, steps=4000)
-    ke = ke_series(amd, storage, u0)
-    err = float(np.max(np.abs(ke - ke_fp64) / ke_fp64))
-    print(f"384^3 {storage}: max relative KE error over 4000 steps {err:.3e}", flush=True)
-    assert err < tol
-    assert ke_fp64[-1] < 0.2 * ke_fp64[0]  # the turbulence decays
+    print_case("full-size-384-ke-enstrophy-mass", shape=SHAPE, storage=storage, steps=4000)
+    got = series(amd, storage, u0)
+    err = np.max(np.abs(got - fp64_series) / np.abs(fp64_series), axis=0)
+    print(f"384^3 {storage}: max relative error over 4000 steps: KE {err[0]:.3e}, enstrophy {err[1]:.3e}, "
+          f"mass {err[2]:.3e}", flush=True)
+    assert err[0] < tol[0]
+    assert err[1] < tol[1]
+    assert err[2] < tol[2]
+    ke, ens = fp64_series[:, 0], fp64_series[:, 1]
+    assert ke[-1] < 0.2 * ke[0] and ens[-1] < ens[0]  # the turbulence decays
