@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 
@@ -626,6 +627,53 @@ int launch_health(d3q19_lattice* L) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Allocation-free read-back of the moments to host memory.  The stateful
+// d3q19_read_moments stages host outputs through a device buffer it
+// allocates per call; this variant stages them through `scratch`, a
+// fixed-size device array of the library (one per GPU, alive until the
+// process ends), so it makes no hipMalloc/hipFree and no allocating call.
+// The array's address on a device is looked up once per device with
+// hipGetSymbolAddress (in host code `scratch` names a host-side stand-in, not
+// device memory) and kept in g_scratch_ptr; only that pointer is passed to
+// kernels and copies.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)D3Q19_NOALLOC_SCRATCH_BYTES;
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+// Device address of `scratch` on device `dev` (looked up on first use, with
+// that device current, then cached).
+int scratch_on(int dev, void** out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(D3Q19_EINVAL, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        int saved = 0;
+        HIPCHK(hipGetDevice(&saved));
+        HIPCHK(hipSetDevice(dev));
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        const hipError_t r = hipSetDevice(saved);
+        if (e != hipSuccess || !ptr)
+            return fail(D3Q19_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        HIPCHK(r);
+        g_scratch_ptr[dev] = ptr;
+    }
+    *out = g_scratch_ptr[dev];
+    return D3Q19_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* d3q19_last_error(void) { return g_err.c_str(); }
@@ -916,6 +964,35 @@ int d3q19_read_moments(d3q19_lattice* L, void* rho, void* u, int32_t dt, int32_t
         if (rho) HIPCHK(hipMemcpyAsync(rho, drho, nc * es, hipMemcpyDeviceToHost, L->stream));
         if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * nc * es, hipMemcpyDeviceToHost, L->stream));
     }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+int d3q19_read_moments_noalloc(d3q19_lattice* L, void* rho, void* u, int32_t dt) {
+    if (int e = check_lat(L)) return e;
+    if (int e = check_dtype(dt)) return e;
+    if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (!rho && !u) return D3Q19_OK;
+    const size_t nc = L->ncell;
+    const size_t es = dt == D3Q19_F64 ? 8 : 4;
+    const size_t need = 4 * nc * es;  // rho (ncell) then u (3, ncell)
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(D3Q19_EINVAL, "read_moments_noalloc stages rho and u (" + std::to_string(need) +
+                                      " bytes) in a fixed " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                      "-byte device buffer: lattice too large, use d3q19_read_moments");
+    void* base = nullptr;
+    if (int e = scratch_on(L->cfg.device, &base)) return e;
+    char* drho = (char*)base;
+    char* du = drho + nc * es;
+    // every staged element is written by the read-back kernel: nothing to clear
+    int rc = dt == D3Q19_F64
+                 ? D3Q19_DISPATCH(L, launch_readback, L, (double*)nullptr, rho ? (double*)drho : (double*)nullptr,
+                                  u ? (double*)du : (double*)nullptr)
+                 : D3Q19_DISPATCH(L, launch_readback, L, (float*)nullptr, rho ? (float*)drho : (float*)nullptr,
+                                  u ? (float*)du : (float*)nullptr);
+    if (rc) return rc;
+    if (rho) HIPCHK(hipMemcpyAsync(rho, drho, nc * es, hipMemcpyDeviceToHost, L->stream));
+    if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * nc * es, hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     return D3Q19_OK;
 }

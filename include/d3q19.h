@@ -116,6 +116,18 @@ int d3q19_read_populations(d3q19_lattice* lat, void* f, int32_t dtype, int32_t o
 /* density_field(f^n) and macroscopic_velocity(f^n, force) (bgk.py:129-142).
    Either output may be NULL. */
 int d3q19_read_moments(d3q19_lattice* lat, void* rho, void* u, int32_t dtype, int32_t on_device);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (tight capture/serving loops, harnesses that forbid hipMalloc):
+ * d3q19_read_moments into HOST rho (Nx,Ny,Nz) and/or u (3,Nx,Ny,Nz), staged
+ * through a fixed device buffer of the library (D3Q19_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends) instead of a per-call allocation;
+ * it calls no hipMalloc/hipFree and no allocating routine.  Limit:
+ * 4 * ncell * sizeof(dtype) <= D3Q19_NOALLOC_SCRATCH_BYTES (FP64: 524,288
+ * cells, FP32: 1,048,576), else D3Q19_EINVAL.  All calls on one GPU share
+ * the buffer: calls that overlap on one GPU (from several threads or
+ * streams) race.  Synchronises the lattice's stream. */
+#define D3Q19_NOALLOC_SCRATCH_BYTES (16u * 1024u * 1024u)
+int d3q19_read_moments_noalloc(d3q19_lattice* lat, void* rho, void* u, int32_t dtype);
 /* FP64 reductions over fluid cells of f^n: out[0] mass, out[1..3] momentum
    sum(c f), out[4] kinetic energy 0.5 sum(rho |u|^2) with the read-back u. */
 int d3q19_diagnostics(d3q19_lattice* lat, double out[5]);
