@@ -372,6 +372,47 @@ def study_cavity(amd, torch, n=256, steps=1000, warmup=20, lid=0.05):
     return out
 
 
+def study_transient(amd, torch, n=256, steps=300, tau=0.55, u0=0.04):
+    """Per-step duration of the fused step right after set-up, FP32 and FP16
+    storage, and the shader clock along the way.  For each storage: one
+    lattice stepped back to back with an event after every step (durations),
+    then a second one stepped in pairs with a clock reading after each pair."""
+    from lattice_boltzmann_d3q19_amd.initial import taylor_green
+
+    shape = (n, n, n)
+    rho0, uu = taylor_green(shape, u0)
+    clock = ShaderClock(torch)
+    stream = torch.cuda.current_stream()
+    out = {"study": "transient", "workload": f"taylor-green-{n}^3-periodic-tau{tau}", "steps": steps,
+           "idle_sclk_MHz": clock()}
+    copy_peak_gbs(amd, torch)  # the headline bench's first phase
+    for storage in ("f32", "f16"):
+        lat = amd.Lattice(shape, tau, storage=storage).set_equilibrium(rho0, uu)
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+        ev[0].record(stream)
+        for k in range(steps):
+            lat.step(1)
+            ev[k + 1].record(stream)
+        torch.cuda.synchronize()
+        us = [round(ev[k].elapsed_time(ev[k + 1]) * 1e3, 1) for k in range(steps)]
+        lat.close()
+        lat = amd.Lattice(shape, tau, storage=storage).set_equilibrium(rho0, uu)
+        torch.cuda.synchronize()
+        clk = []
+        for k in range(0, steps, 2):
+            lat.step(2)
+            clk.append(clock())
+        lat.close()
+        out[storage] = {"step_us": us, "sclk_MHz_every_2_steps": clk,
+                        "mean_us_steps_1_10": sum(us[:10]) / 10, "mean_us_steps_11_40": sum(us[10:40]) / 30,
+                        "mean_us_last_100": sum(us[-100:]) / 100, "min_sclk_MHz": min(clk)}
+        print(f"[study] transient {storage}: steps 1-10 {out[storage]['mean_us_steps_1_10']:.1f} us, "
+              f"11-40 {out[storage]['mean_us_steps_11_40']:.1f} us, last 100 {out[storage]['mean_us_last_100']:.1f} us, "
+              f"min sclk {min(clk)} MHz", file=sys.stderr, flush=True)
+    return out
+
+
 def study_mms(levels=(32, 64, 128, 256, 512, 1024), storages=("f64", "f32")):
     """Forced Taylor-Green MMS ladder on the fused step (solution verification):
     RMS velocity error at t_final per level and storage, observed orders."""
@@ -505,7 +546,8 @@ def config_baselines(amd, torch):
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (headline default 5000; --study cavity: steps per case, default 1000)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--tau", type=float, default=0.55)
@@ -517,7 +559,8 @@ def main() -> int:
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-me16", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms", "cavity"), default=None,
+    ap.add_argument("--study", choices=("turbulence", "inverse", "channel", "mms", "cavity", "transient"),
+                    default=None,
                     help="run a validation study instead of the headline benchmark")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -545,8 +588,13 @@ def main() -> int:
         print(json.dumps(study_mms()), flush=True)
         return 0
     if args.study == "cavity":
-        print(json.dumps(study_cavity(amd, torch)), flush=True)
+        print(json.dumps(study_cavity(amd, torch, steps=args.steps or 1000)), flush=True)
         return 0
+    if args.study == "transient":
+        print(json.dumps(study_transient(amd, torch)), flush=True)
+        return 0
+    if args.steps is None:
+        args.steps = 5000
     n = args.size
     shape = (n, n, n)
     ncell = n ** 3

@@ -339,6 +339,7 @@ Params<C> adj_params(const d3q19_adjoint* A, size_t es) {
         p.wbot[a] = (C)A->cfg.wall_velocity_bottom[a];
         for (int k = 0; k < 6; ++k) p.face_u[k][a] = (C)A->cfg.face_velocity[k][a];
     }
+    set_zwall_terms(p);
     return p;
 }
 

@@ -119,6 +119,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.brink = (const C*)L->brink;
     p.solid = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->solid : nullptr;
     p.local = L->local ? 1 : 0;
+    set_zwall_terms(p);
     p.seed = (uint32_t)L->steps * 0x9e3779b9u + 0x7f4a7c15u;
     return p;
 }

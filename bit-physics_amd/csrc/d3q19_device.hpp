@@ -33,7 +33,7 @@ __host__ __device__ constexpr T wgt(int i) {
 // c_i . (x, y, z) with the zero components left out (every component of c_i
 // is 0 or +-1), so no product with zero is formed.
 template <class C>
-__device__ __forceinline__ C cdot(int i, C x, C y, C z) {
+__host__ __device__ __forceinline__ C cdot(int i, C x, C y, C z) {
     C r = C(0);
     bool any = false;
     if (CX[i] != 0) {
@@ -189,7 +189,20 @@ template <class C> struct Params {
     // correctly rounded value of its exact product: om w_k, 3 om w_k,
     // 4.5 om w_k, 3 pref w_k, 9 pref w_k
     C k_base[3], k_odd[3], k_quad[3], k_fodd[3], k_fquad[3];
+    // half-way z walls: the moving-wall term 6 w_i (c_i . u_face) of each
+    // direction bounced at the z = 0 face (c_z > 0, [0]) and the z = nz - 1
+    // face (c_z < 0, [1]), formed as the general path forms it
+    C zterm[2][19];
 };
+
+// Fill the z-wall terms of p from its face velocities (host).
+template <class C>
+inline void set_zwall_terms(Params<C>& p) {
+    for (int i = 0; i < 19; ++i) {
+        p.zterm[0][i] = CZ[i] > 0 ? C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[4][0], p.face_u[4][1], p.face_u[4][2]) : C(0);
+        p.zterm[1][i] = CZ[i] < 0 ? C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[5][0], p.face_u[5][1], p.face_u[5][2]) : C(0);
+    }
+}
 
 // Fill the collision coefficients of p for relaxation time tau (host).
 template <class C>
@@ -266,6 +279,29 @@ __device__ __forceinline__ const char* row_of(const void* base, const Params<C>&
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
 }
 
+// U: the address of a direction segment of a row, wavefront-uniform (the row
+// grid: every wavefront lies in one row), passed through readfirstlane.  The
+// compiler cannot reassociate the lane's 32-bit offset into the opaque
+// uniform base, so each access takes the SGPR-base + 32-bit VGPR-offset form.
+// Without it LLVM shares (row + z s) over the 19 stores of a cell and forms
+// one 64-bit add per store (v_lshl_add_u64: 18 VALU per cell), and the
+// half-way kernel does the same for nine of its loads.  Never for the linear
+// grid of short-row lattices, where every lane has its own row; the kernels
+// choose it per storage (k_step).
+template <bool U>
+__device__ __forceinline__ const char* seg_base(const char* p) {
+    if constexpr (U) {
+        const uint64_t v = (uint64_t)p;
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+        // rebuilt as a global-memory pointer, so the accesses stay global_*
+        typedef __attribute__((address_space(1))) const char gchar;
+        return (const char*)(gchar*)(((uint64_t)hi << 32) | lo);
+    } else {
+        return p;
+    }
+}
+
 // Population i of a row at byte offset zoff; NT: with the nontemporal hint.
 // In the fused step the hint depends on the
 // direction.  A z-shifted pull (c_z != 0) straddles a line that the
@@ -287,27 +323,27 @@ __device__ __forceinline__ typename Codec<Tag>::C ld_at(const char* addr, int i)
         return Codec<Tag>::dec(*a, i);
 }
 
-template <class Tag, bool NT = false>
+template <class Tag, bool NT = false, bool U = false>
 __device__ __forceinline__ typename Codec<Tag>::C ld(const char* row, int i, uint32_t dir_bytes, uint32_t zoff) {
-    return ld_at<Tag, NT>(row + (uint32_t)i * dir_bytes + zoff, i);
+    return ld_at<Tag, NT>(seg_base<U>(row + (uint32_t)i * dir_bytes) + zoff, i);
 }
 
 // Stores are nontemporal: each population is written once per step and read
 // in the next (plain FP32 stores measured 12% slower at 256^3).
-template <class Tag>
+template <class Tag, bool U = false>
 __device__ __forceinline__ void st(char* row, int i, uint32_t dir_bytes, uint32_t zoff, typename Codec<Tag>::C v) {
     typedef typename Codec<Tag>::raw R;
-    __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)(row + (uint32_t)i * dir_bytes + zoff));
+    __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)(seg_base<U>(row + (uint32_t)i * dir_bytes) + zoff));
 }
 
 // In-place (unstreamed) populations of cell (x, y, z).
-template <class Tag>
+template <class Tag, bool U = false>
 __device__ __forceinline__ void load_local(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                            uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     const char* row = row_of(in, p, x, y);
 #pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = ld<Tag>(row, i, p.dir_bytes, z * ES);
+    for (int i = 0; i < Q; ++i) d[i] = ld<Tag, false, U>(row, i, p.dir_bytes, z * ES);
 }
 
 // 32-bit integer hash (two xorshift-multiply rounds).
@@ -320,30 +356,41 @@ __device__ __forceinline__ uint32_t mix32(uint32_t h) {
     return h;
 }
 
+// The raw value store_local writes into slot i of cell (x, y, z).  For
+// stochastically rounded BF16 the rounding bits come from a hash of (cell,
+// step) and one multiply per population, so the value is a function of the
+// cell, the step and d alone.
 template <class Tag>
-__device__ __forceinline__ void store_local(void* out, const Params<typename Codec<Tag>::C>& p, uint32_t x,
-                                            uint32_t y, uint32_t z, const typename Codec<Tag>::C d[Q]) {
-    constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
-    char* row = (char*)row_of(out, p, x, y);
+__device__ __forceinline__ typename Codec<Tag>::raw raw_of(const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                                          uint32_t y, uint32_t z, typename Codec<Tag>::C v, int i) {
     if constexpr (std::is_same<Tag, TBF16S>::value) {
         // BF16 keeps 8 mantissa bits: nearest rounding of the small deviations
         // biases them the same way every step (kinetic energy of decaying
         // turbulence off by 1.4e-2 at 384^3); stochastic rounding removes the
-        // bias.  The rounding bits come from a hash of (cell, step) and one
-        // multiply per population, so a run is reproducible bit for bit.
+        // bias.  key * odd constant (mod 2^32) is a bijection of the hashed
+        // key, so its top 16 bits are uniform.
         const uint32_t key = mix32(((x * p.ny + y) * p.nz + z) ^ p.seed);
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            // key * odd constant (mod 2^32) is a bijection of the hashed key,
-            // so its top 16 bits are uniform
-            const uint32_t r = (key * (0x9e3779b1u * (uint32_t)(2 * i + 1))) >> 16;
-            __builtin_nontemporal_store(Codec<TBF16>::enc_sr(d[i], r),
-                                        (uint16_t*)(row + (uint32_t)i * p.dir_bytes + z * ES));
-        }
+        const uint32_t r = (key * (0x9e3779b1u * (uint32_t)(2 * i + 1))) >> 16;
+        return Codec<TBF16>::enc_sr(v, r);
     } else {
-#pragma unroll
-        for (int i = 0; i < Q; ++i) st<Tag>(row, i, p.dir_bytes, z * ES, d[i]);
+        return Codec<Tag>::enc(v, i);
     }
+}
+
+template <class Tag, bool U = false>
+__device__ __forceinline__ void store_local(void* out, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                            uint32_t y, uint32_t z, const typename Codec<Tag>::C d[Q]) {
+    typedef typename Codec<Tag>::raw R;
+    constexpr uint32_t ES = sizeof(R);
+    char* row = (char*)row_of(out, p, x, y);
+    uint32_t zoff = z * ES;
+    // an opaque 32-bit lane offset: with the forced collision in the same
+    // kernel LLVM otherwise widened it to 64 bits and added it per store
+    if constexpr (U) asm volatile("" : "+v"(zoff));
+#pragma unroll
+    for (int i = 0; i < Q; ++i)
+        __builtin_nontemporal_store(raw_of<Tag>(p, x, y, z, d[i], i),
+                                    (R*)(seg_base<U>(row + (uint32_t)i * p.dir_bytes) + zoff));
 }
 
 // Reference y-wall treatment (reference/bgk.py:190-215) applied in registers
@@ -402,7 +449,7 @@ __device__ __forceinline__ int halfway_face(int i, uint32_t wm, uint32_t x, uint
 }
 
 // Periodic pull of all 19 directions (the load hints as described at ld).
-template <class Tag, bool NT>
+template <class Tag, bool NT, bool U = false>
 __device__ __forceinline__ void pull_periodic(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t z,
                                               const uint32_t xs[3], const uint32_t ys[3], const uint32_t zs[3],
                                               typename Codec<Tag>::C d[Q]) {
@@ -412,9 +459,9 @@ __device__ __forceinline__ void pull_periodic(const void* in, const Params<typen
     for (int i = 0; i < Q; ++i) {
         const char* row = row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]);
         if (CZ[i] == 0)
-            d[i] = ld<Tag, true>(row, i, DB, z * ES);
+            d[i] = ld<Tag, true, U>(row, i, DB, z * ES);
         else
-            d[i] = ld<Tag, NT>(row, i, DB, zs[1 - CZ[i]] * ES);
+            d[i] = ld<Tag, NT, U>(row, i, DB, zs[1 - CZ[i]] * ES);
     }
 }
 
@@ -424,134 +471,212 @@ __device__ __forceinline__ void pull_periodic(const void* in, const Params<typen
 // bounce-back, from the stored post-collision deviations.  Direction i pulls
 // from row (x - cx, y - cy) at z - cz.
 // ---------------------------------------------------------------------------
-template <class Tag, int BC, bool NT = false>
-__device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
-                                     uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
+
+// Source rows and z positions of a cell's pull: index 0/1/2 along an axis for
+// velocity component +1/0/-1, wrapped periodically.
+#define D3Q19_PULL_SOURCES(p, x, y, z)                                                   \
+    const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};     \
+    const uint32_t ys[3] = {y == 0 ? p.ny - 1 : y - 1, y, y + 1 == p.ny ? 0 : y + 1};     \
+    const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1}
+
+// Half-way mode.  A lattice without solid cells splits its rows in two:
+// rows on an x or y wall face (edge rows) and the others (interior rows).
+// Whether a direction crosses an x or y face is the same for every lane of a
+// row, and a z face is met only by the lanes at z = 0 and z = nz - 1, so
+// neither kind needs a per-lane choice of address for the x and y faces.
+// Lattices with solids take the general path (per-direction selects).
+template <class C>
+__device__ __forceinline__ bool halfway_edge_row(const Params<C>& p, uint32_t x, uint32_t y) {
+    const uint32_t wm = p.wall_mask;
+    return ((wm & 1u) && x == 0) || ((wm & 2u) && x + 1 == p.nx) || ((wm & 4u) && y == 0) ||
+           ((wm & 8u) && y + 1 == p.ny);
+}
+
+// z-wall bounce values, loaded raw.  A lane at z = 0 bounces, for each i with
+// c_z > 0, its own opp(i); a lane at z = nz - 1, for each i with c_z < 0,
+// its own opp(i).  The address is the same in every wall lane of a
+// wavefront (one row, fixed z), so on the row grid it is a scalar base.
+template <class Tag, bool U>
+__device__ __forceinline__ void zwall_fetch(const char* self, const Params<typename Codec<Tag>::C>& p, bool zlo,
+                                            bool zhi, typename Codec<Tag>::raw rlo[Q],
+                                            typename Codec<Tag>::raw rhi[Q]) {
+    typedef typename Codec<Tag>::raw R;
+    const uint32_t DB = p.dir_bytes;
+    if (zlo) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            if (CZ[i] > 0) rlo[i] = *(const R*)seg_base<U>(self + (uint32_t)OPP[i] * DB);
+    }
+    if (zhi) {
+        const uint32_t zo = (p.nz - 1) * (uint32_t)sizeof(R);
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            if (CZ[i] < 0) rhi[i] = *(const R*)(seg_base<U>(self + (uint32_t)OPP[i] * DB) + zo);
+    }
+}
+
+// The z-wall lanes' bounced directions: opp(i) decodes with index i (same
+// weight class), plus the face's precomputed moving-wall term.  Applied
+// after any x/y bounce: at an edge the z face is the one that counts.
+template <class Tag>
+__device__ __forceinline__ void zwall_apply(const Params<typename Codec<Tag>::C>& p, bool zlo, bool zhi,
+                                            const typename Codec<Tag>::raw rlo[Q],
+                                            const typename Codec<Tag>::raw rhi[Q], typename Codec<Tag>::C d[Q]) {
+#pragma clang fp contract(off)
+    if (zlo) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            if (CZ[i] > 0) d[i] = Codec<Tag>::dec(rlo[i], i) + p.zterm[0][i];
+    }
+    if (zhi) {
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            if (CZ[i] < 0) d[i] = Codec<Tag>::dec(rhi[i], i) + p.zterm[1][i];
+    }
+}
+
+// Interior rows: the periodic pull plus the z-wall bounce.  For 16-bit
+// storage the wall lanes load their bounce values before the pull (loads
+// complete in issue order, so the pull's wait covers them); for 4- and
+// 8-byte storage after it (the earlier order ran FP32 1.7% slower).
+template <class Tag, bool NT = false, bool U = false>
+__device__ __forceinline__ void pull_halfway_fast(const void* in, const Params<typename Codec<Tag>::C>& p,
+                                                  uint32_t x, uint32_t y, uint32_t z,
+                                                  typename Codec<Tag>::C d[Q]) {
+    typedef typename Codec<Tag>::raw R;
+    D3Q19_PULL_SOURCES(p, x, y, z);
+    const uint32_t wm = p.wall_mask;
+    const char* self = row_of(in, p, x, y);
+    const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
+    R rlo[Q], rhi[Q];
+    if constexpr (sizeof(R) == 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+    pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
+    if constexpr (sizeof(R) != 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+    zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
+}
+
+// Edge rows: per direction, a wavefront-uniform choice between the pull
+// source and the cell's own opposite population (bounce from an x or y
+// face, moving-wall term of the face that counts: y over x), then the
+// z-wall bounce as for interior rows.
+template <class Tag, bool NT = false, bool U = false>
+__device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<typename Codec<Tag>::C>& p,
+                                                  uint32_t x, uint32_t y, uint32_t z,
+                                                  typename Codec<Tag>::C d[Q]) {
+#pragma clang fp contract(off)
+    typedef typename Codec<Tag>::C C;
+    typedef typename Codec<Tag>::raw R;
+    constexpr uint32_t ES = sizeof(R);
+    D3Q19_PULL_SOURCES(p, x, y, z);
+    const uint32_t DB = p.dir_bytes;
+    const uint32_t wm = p.wall_mask;
+    const char* self = row_of(in, p, x, y);
+    const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
+    R rlo[Q], rhi[Q];
+    if constexpr (sizeof(R) == 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const bool wx = (CX[i] > 0 && x == 0 && (wm & 1u)) || (CX[i] < 0 && x + 1 == p.nx && (wm & 2u));
+        const bool wy = (CY[i] > 0 && y == 0 && (wm & 4u)) || (CY[i] < 0 && y + 1 == p.ny && (wm & 8u));
+        const bool cross = wx || wy;
+        const char* base = cross ? self + (uint32_t)OPP[i] * DB
+                                 : row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]) + (uint32_t)i * DB;
+        const uint32_t off = (CZ[i] == 0 || cross) ? z * ES : zs[1 - CZ[i]] * ES;
+        const C v = CZ[i] == 0 ? ld_at<Tag, true>(seg_base<U>(base) + off, i)
+                               : ld_at<Tag, NT>(seg_base<U>(base) + off, i);
+        if (cross) {
+            const int f = wy ? (CY[i] > 0 ? 2 : 3) : (CX[i] > 0 ? 0 : 1);
+            d[i] = v + C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[f][0], p.face_u[f][1], p.face_u[f][2]);
+        } else {
+            d[i] = v;
+        }
+    }
+    if constexpr (sizeof(R) != 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+    zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
+}
+
+// Half-way general pull (lattices with solid cells): per-direction selects
+// between the pull source and the cell's own opposite population.
+template <class Tag, bool NT = false>
+__device__ __forceinline__ void pull_halfway_general(const void* in, const Params<typename Codec<Tag>::C>& p,
+                                                     uint32_t x, uint32_t y, uint32_t z,
+                                                     typename Codec<Tag>::C d[Q]) {
 #pragma clang fp contract(off)
     typedef typename Codec<Tag>::C C;
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
-    const uint32_t xs[3] = {x == 0 ? p.nx - 1 : x - 1, x, x + 1 == p.nx ? 0 : x + 1};
-    const uint32_t ys[3] = {y == 0 ? p.ny - 1 : y - 1, y, y + 1 == p.ny ? 0 : y + 1};
-    const uint32_t zs[3] = {z == 0 ? p.nz - 1 : z - 1, z, z + 1 == p.nz ? 0 : z + 1};
+    D3Q19_PULL_SOURCES(p, x, y, z);
     const uint32_t DB = p.dir_bytes;
-    // source index 0/1/2 along an axis for velocity component +1/0/-1
-    if constexpr (BC != BC_HALFWAY) pull_periodic<Tag, NT>(in, p, z, xs, ys, zs, d);
-    if constexpr (BC == BC_YWALLS) ywall_swap<C>(p, y, d);
+    const uint32_t wm = p.wall_mask;
+    const char* self = row_of(in, p, x, y);
+    // Solid source cells: all flags are loaded before any population, so
+    // that no population load waits on a flag (none when there are no
+    // solids: the branch is uniform).
+    bool solid_src[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) solid_src[i] = false;
+    if (p.solid) {
+#pragma unroll
+        for (int i = 1; i < Q; ++i)
+            solid_src[i] = p.solid[(xs[1 - CX[i]] * p.ny + ys[1 - CY[i]]) * p.nz + zs[1 - CZ[i]]] != 0;
+    }
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        // The pull crosses a wall face; the face that counts for the
+        // moving-wall term is the last in face order (z over y over x,
+        // as halfway_face), with compile-time face indices.
+        const bool wx = (CX[i] > 0 && x == 0 && (wm & 1u)) || (CX[i] < 0 && x + 1 == p.nx && (wm & 2u));
+        const bool wy = (CY[i] > 0 && y == 0 && (wm & 4u)) || (CY[i] < 0 && y + 1 == p.ny && (wm & 8u));
+        const bool wz = (CZ[i] > 0 && z == 0 && (wm & 16u)) || (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u));
+        const bool wall = wx || wy || wz;
+        C u0 = C(0), u1 = C(0), u2 = C(0);
+        if (CX[i] != 0) {
+            const int f = CX[i] > 0 ? 0 : 1;
+            u0 = wx ? p.face_u[f][0] : u0;
+            u1 = wx ? p.face_u[f][1] : u1;
+            u2 = wx ? p.face_u[f][2] : u2;
+        }
+        if (CY[i] != 0) {
+            const int f = CY[i] > 0 ? 2 : 3;
+            u0 = wy ? p.face_u[f][0] : u0;
+            u1 = wy ? p.face_u[f][1] : u1;
+            u2 = wy ? p.face_u[f][2] : u2;
+        }
+        if (CZ[i] != 0) {
+            const int f = CZ[i] > 0 ? 4 : 5;
+            u0 = wz ? p.face_u[f][0] : u0;
+            u1 = wz ? p.face_u[f][1] : u1;
+            u2 = wz ? p.face_u[f][2] : u2;
+        }
+        // Bounce-back (a wall face, or a solid source cell) reads the
+        // opposite population of the cell itself, else the pull source.
+        // One load per direction from a selected address: a load in one
+        // arm of a branch would have to complete (and, for 16-bit
+        // storage, be decoded) before the arms join, serialising the 19
+        // loads.  opp(i) has the weight class of i, so the value decodes
+        // with index i either way.
+        const bool back = wall || solid_src[i];
+        const char* src = back ? self + (uint32_t)OPP[i] * DB + z * ES
+                               : row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]) + (uint32_t)i * DB +
+                                     zs[1 - CZ[i]] * ES;
+        const C v = CZ[i] == 0 ? ld_at<Tag, true>(src, i) : ld_at<Tag, NT>(src, i);
+        d[i] = wall ? v + C(6) * wgt<C>(i) * cdot<C>(i, u0, u1, u2) : v;
+    }
+}
+
+template <class Tag, int BC, bool NT = false, bool U = false>
+__device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
+                                     uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     if constexpr (BC == BC_HALFWAY) {
-        const uint32_t wm = p.wall_mask;
-        const char* self = row_of(in, p, x, y);
-        // A row off the x and y wall faces, in a lattice without solids, can
-        // only meet the z faces, and only in its lanes at z = 0 and z = nz - 1
-        // (the row is wavefront-uniform, so this branch is too).  Such a row
-        // takes the periodic pull and then bounces those lanes' z-crossing
-        // directions back: the same loads and values as the general path
-        // below, without its per-direction selects in every lane.
-        const bool edge_row = ((wm & 1u) && x == 0) || ((wm & 2u) && x + 1 == p.nx) ||
-                              ((wm & 4u) && y == 0) || ((wm & 8u) && y + 1 == p.ny);
-        typedef typename Codec<Tag>::raw R;
-        if constexpr (sizeof(R) == 2) {
-            // 16-bit storage: the wall lanes load their bounce values (raw)
-            // before the periodic pull and decode them after it.  Loads
-            // complete in issue order, so the wait for the pull covers them
-            // and the wavefront waits for memory once, not twice (256^3
-            // lid-driven cavity, FP16: 75.0 against 69.2 GLUPS for the patch
-            // after the pull; bit-identical).  With nz = 1 a lane sits at both
-            // z walls; such lattices take the general path.
-            if (!edge_row && !p.solid && p.nz > 1) {
-                const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
-                R rb[Q];
-                if (zlo || zhi) {
-                    // pair (i, opp(i)) with c_z(i) > 0: at z = 0, i bounces
-                    // opp(i) of the cell; at z = nz - 1, opp(i) bounces i
-#pragma unroll
-                    for (int i = 0; i < Q; ++i)
-                        if (CZ[i] > 0) rb[i] = *(const R*)(self + (uint32_t)(zlo ? OPP[i] : i) * DB + z * ES);
-                }
-                pull_periodic<Tag, NT>(in, p, z, xs, ys, zs, d);
-                if (zlo || zhi) {
-#pragma unroll
-                    for (int i = 0; i < Q; ++i) {
-                        if (CZ[i] <= 0) continue;
-                        const int o = OPP[i];
-                        const C v = Codec<Tag>::dec(rb[i], i);  // opp(i) shares i's weight class
-                        const C vlo = v + C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[4][0], p.face_u[4][1], p.face_u[4][2]);
-                        const C vhi = v + C(6) * wgt<C>(o) * cdot<C>(o, p.face_u[5][0], p.face_u[5][1], p.face_u[5][2]);
-                        d[i] = zlo ? vlo : d[i];
-                        d[o] = zhi ? vhi : d[o];
-                    }
-                }
-                return;
-            }
-        }
-        // 4- and 8-byte storage: the patch after the pull (the form above ran
-        // FP32 1.7% slower: cavity 39.75 against 40.47 GLUPS).
-        if (sizeof(R) != 2 && !edge_row && !p.solid) {
-            pull_periodic<Tag, NT>(in, p, z, xs, ys, zs, d);
-            const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
-            if (zlo || zhi) {
-#pragma unroll
-                for (int i = 0; i < Q; ++i) {
-                    if (CZ[i] == 0 || !(CZ[i] > 0 ? zlo : zhi)) continue;
-                    const int f = CZ[i] > 0 ? 4 : 5;
-                    const C v = ld_at<Tag, NT>(self + (uint32_t)OPP[i] * DB + z * ES, i);
-                    d[i] = v + C(6) * wgt<C>(i) * cdot<C>(i, p.face_u[f][0], p.face_u[f][1], p.face_u[f][2]);
-                }
-            }
-            return;
-        }
-        // Solid source cells: all flags are loaded before any population, so
-        // that no population load waits on a flag (none when there are no
-        // solids: the branch is uniform).
-        bool solid_src[Q];
-#pragma unroll
-        for (int i = 0; i < Q; ++i) solid_src[i] = false;
-        if (p.solid) {
-#pragma unroll
-            for (int i = 1; i < Q; ++i)
-                solid_src[i] = p.solid[(xs[1 - CX[i]] * p.ny + ys[1 - CY[i]]) * p.nz + zs[1 - CZ[i]]] != 0;
-        }
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            // The pull crosses a wall face; the face that counts for the
-            // moving-wall term is the last in face order (z over y over x,
-            // as halfway_face), with compile-time face indices.
-            const bool wx = (CX[i] > 0 && x == 0 && (wm & 1u)) || (CX[i] < 0 && x + 1 == p.nx && (wm & 2u));
-            const bool wy = (CY[i] > 0 && y == 0 && (wm & 4u)) || (CY[i] < 0 && y + 1 == p.ny && (wm & 8u));
-            const bool wz = (CZ[i] > 0 && z == 0 && (wm & 16u)) || (CZ[i] < 0 && z + 1 == p.nz && (wm & 32u));
-            const bool wall = wx || wy || wz;
-            C u0 = C(0), u1 = C(0), u2 = C(0);
-            if (CX[i] != 0) {
-                const int f = CX[i] > 0 ? 0 : 1;
-                u0 = wx ? p.face_u[f][0] : u0;
-                u1 = wx ? p.face_u[f][1] : u1;
-                u2 = wx ? p.face_u[f][2] : u2;
-            }
-            if (CY[i] != 0) {
-                const int f = CY[i] > 0 ? 2 : 3;
-                u0 = wy ? p.face_u[f][0] : u0;
-                u1 = wy ? p.face_u[f][1] : u1;
-                u2 = wy ? p.face_u[f][2] : u2;
-            }
-            if (CZ[i] != 0) {
-                const int f = CZ[i] > 0 ? 4 : 5;
-                u0 = wz ? p.face_u[f][0] : u0;
-                u1 = wz ? p.face_u[f][1] : u1;
-                u2 = wz ? p.face_u[f][2] : u2;
-            }
-            // Bounce-back (a wall face, or a solid source cell) reads the
-            // opposite population of the cell itself, else the pull source.
-            // One load per direction from a selected address: a load in one
-            // arm of a branch would have to complete (and, for 16-bit
-            // storage, be decoded) before the arms join, serialising the 19
-            // loads.  opp(i) has the weight class of i, so the value decodes
-            // with index i either way.
-            const bool back = wall || solid_src[i];
-            const char* src = back ? self + (uint32_t)OPP[i] * DB + z * ES
-                                   : row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]) + (uint32_t)i * DB +
-                                         zs[1 - CZ[i]] * ES;
-            const C v = CZ[i] == 0 ? ld_at<Tag, true>(src, i) : ld_at<Tag, NT>(src, i);
-            d[i] = wall ? v + C(6) * wgt<C>(i) * cdot<C>(i, u0, u1, u2) : v;
-        }
+        if (p.solid)
+            pull_halfway_general<Tag, NT>(in, p, x, y, z, d);
+        else if (halfway_edge_row(p, x, y))
+            pull_halfway_edge<Tag, NT, U>(in, p, x, y, z, d);
+        else
+            pull_halfway_fast<Tag, NT, U>(in, p, x, y, z, d);
+    } else {
+        D3Q19_PULL_SOURCES(p, x, y, z);
+        pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
+        if constexpr (BC == BC_YWALLS) ywall_swap<typename Codec<Tag>::C>(p, y, d);
     }
 }
 

@@ -7,6 +7,18 @@ namespace d3q19 {
 
 constexpr int BLOCK = 256;
 
+// rho and u of the step's pre-collision state, when asked for (FP32).
+template <class C>
+__device__ __forceinline__ void step_moments_out(float* rho_out, float* u_out, uint32_t ncell, uint32_t cell, C rho,
+                                                 C ux, C uy, C uz) {
+    if (rho_out) rho_out[cell] = (float)rho;
+    if (u_out) {
+        u_out[cell] = (float)ux;
+        u_out[(size_t)ncell + cell] = (float)uy;
+        u_out[2 * (size_t)ncell + cell] = (float)uz;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // The hot path: one fused pull-stream / boundary / moments / BGK+Guo pass.
 // Reads the post-collision state of step n-1, writes that of step n, and
@@ -20,22 +32,45 @@ static __global__ void __launch_bounds__(fused_block<Tag>())
     uint32_t x, y, z;
     if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
+    // Wavefront-uniform segment bases (seg_base) for 16-bit storage on the
+    // row grid; for FP32 the same store form made the periodic step slower
+    // (256^3: 40.4 -> 38.4 GLUPS), so 4- and 8-byte storage keep the
+    // compiler's addressing.
+    constexpr bool U = ROWS && sizeof(typename Codec<Tag>::raw) == 2;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
+    if constexpr (BC == BC_HALFWAY) {
+        // Without solids: interior rows and edge rows each with a collision
+        // and store of their own, compiled apart from each other and from
+        // the general path below (fewer live scalar registers in the
+        // interior rows' code).  Measured against one shared path, 256^3,
+        // GLUPS (FP16 / FP32): lid-driven cavity 73.0 -> 75.9 / 40.5 -> 40.5,
+        // half-way z walls 73.2 -> 76.4 / 40.1 -> 40.4; together with the
+        // uniform segment bases, reference y-walls in FP16 79.6 -> 82.9.
+        if (!p.solid) {
+            if (!halfway_edge_row(p, x, y)) {
+                pull_halfway_fast<Tag, false, U>(in, p, x, y, z, d);
+                collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+                store_local<Tag, U>(out, p, x, y, z, d);
+                step_moments_out(rho_out, u_out, p.ncell, cell, rho, ux, uy, uz);
+                return;
+            }
+            pull_halfway_edge<Tag, false, U>(in, p, x, y, z, d);
+            collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+            store_local<Tag, U>(out, p, x, y, z, d);
+            step_moments_out(rho_out, u_out, p.ncell, cell, rho, ux, uy, uz);
+            return;
+        }
+    }
     if (BC == BC_HALFWAY && p.solid && p.solid[cell]) {
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-        pull<Tag, BC>(in, p, x, y, z, d);
+        pull<Tag, BC, false, U>(in, p, x, y, z, d);
         collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
     }
-    store_local<Tag>(out, p, x, y, z, d);
-    if (rho_out) rho_out[cell] = (float)rho;
-    if (u_out) {
-        u_out[cell] = (float)ux;
-        u_out[(size_t)p.ncell + cell] = (float)uy;
-        u_out[2 * (size_t)p.ncell + cell] = (float)uz;
-    }
+    store_local<Tag, U>(out, p, x, y, z, d);
+    step_moments_out(rho_out, u_out, p.ncell, cell, rho, ux, uy, uz);
 }
 
 // Load a reference-layout population field f^n as the lattice state, in
@@ -62,6 +97,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     k_collide_local(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
                     Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
+    constexpr bool U = ROWS && sizeof(typename Codec<Tag>::raw) == 2;  // as k_step
     uint32_t x, y, z;
     if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
@@ -71,10 +107,10 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
-        load_local<Tag>(in, p, x, y, z, d);
+        load_local<Tag, U>(in, p, x, y, z, d);
         collide(d, p, cell, rho, ux, uy, uz);
     }
-    store_local<Tag>(out, p, x, y, z, d);
+    store_local<Tag, U>(out, p, x, y, z, d);
     if (rho_out) rho_out[cell] = (float)rho;
     if (u_out) {
         u_out[cell] = (float)ux;

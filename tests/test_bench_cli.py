@@ -28,5 +28,5 @@ def test_help_lists_the_contract_flags():
     assert r.returncode == 0
     for flag in ("--gpus", "--steps", "--warmup", "--study"):
         assert flag in r.stdout
-    for study in ("turbulence", "inverse", "channel", "mms", "cavity"):
+    for study in ("turbulence", "inverse", "channel", "mms", "cavity", "transient"):
         assert study in r.stdout
