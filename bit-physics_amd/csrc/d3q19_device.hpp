@@ -538,27 +538,44 @@ __device__ __forceinline__ void zwall_apply(const Params<typename Codec<Tag>::C>
 // Interior rows: the periodic pull plus the z-wall bounce.  For 16-bit
 // storage the wall lanes load their bounce values before the pull (loads
 // complete in issue order, so the pull's wait covers them); for 4- and
-// 8-byte storage after it (the earlier order ran FP32 1.7% slower).
+// 8-byte storage after it, in one block for both faces (the earlier order
+// ran FP32 1.7% slower at 256^3; two blocks, as for 16-bit storage, made
+// the latency-bound 64^3 cavity 11% slower).
 template <class Tag, bool NT = false, bool U = false>
 __device__ __forceinline__ void pull_halfway_fast(const void* in, const Params<typename Codec<Tag>::C>& p,
                                                   uint32_t x, uint32_t y, uint32_t z,
                                                   typename Codec<Tag>::C d[Q]) {
+#pragma clang fp contract(off)
     typedef typename Codec<Tag>::raw R;
+    constexpr uint32_t ES = sizeof(R);
     D3Q19_PULL_SOURCES(p, x, y, z);
     const uint32_t wm = p.wall_mask;
     const char* self = row_of(in, p, x, y);
     const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
-    R rlo[Q], rhi[Q];
-    if constexpr (sizeof(R) == 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
-    pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
-    if constexpr (sizeof(R) != 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
-    zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
+    if constexpr (sizeof(R) == 2) {
+        R rlo[Q], rhi[Q];
+        zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+        pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
+        zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
+    } else {
+        pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
+        if (zlo || zhi) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                if (CZ[i] == 0 || !(CZ[i] > 0 ? zlo : zhi)) continue;
+                const typename Codec<Tag>::C v = ld_at<Tag, NT>(self + (uint32_t)OPP[i] * p.dir_bytes + z * ES, i);
+                d[i] = v + p.zterm[CZ[i] > 0 ? 0 : 1][i];
+            }
+        }
+    }
 }
 
-// Edge rows: per direction, a wavefront-uniform choice between the pull
-// source and the cell's own opposite population (bounce from an x or y
-// face, moving-wall term of the face that counts: y over x), then the
-// z-wall bounce as for interior rows.
+// Edge rows of 16-bit lattices: per direction, a wavefront-uniform choice
+// between the pull source and the cell's own opposite population (bounce
+// from an x or y face, moving-wall term of the face that counts: y over x),
+// then the z-wall bounce as for interior rows.  (4- and 8-byte storage
+// takes the general path on edge rows: the same at 256^3, and shorter at
+// 64^3, where every row is one wavefront.)
 template <class Tag, bool NT = false, bool U = false>
 __device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<typename Codec<Tag>::C>& p,
                                                   uint32_t x, uint32_t y, uint32_t z,
@@ -573,7 +590,7 @@ __device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<t
     const char* self = row_of(in, p, x, y);
     const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
     R rlo[Q], rhi[Q];
-    if constexpr (sizeof(R) == 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
+    zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         const bool wx = (CX[i] > 0 && x == 0 && (wm & 1u)) || (CX[i] < 0 && x + 1 == p.nx && (wm & 2u));
@@ -591,7 +608,6 @@ __device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<t
             d[i] = v;
         }
     }
-    if constexpr (sizeof(R) != 2) zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
     zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
 }
 
@@ -667,7 +683,8 @@ template <class Tag, int BC, bool NT = false, bool U = false>
 __device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                      uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     if constexpr (BC == BC_HALFWAY) {
-        if (p.solid)
+        constexpr bool short_raw = sizeof(typename Codec<Tag>::raw) == 2;
+        if (p.solid || (!short_raw && halfway_edge_row(p, x, y)))
             pull_halfway_general<Tag, NT>(in, p, x, y, z, d);
         else if (halfway_edge_row(p, x, y))
             pull_halfway_edge<Tag, NT, U>(in, p, x, y, z, d);

@@ -39,14 +39,15 @@ static __global__ void __launch_bounds__(fused_block<Tag>())
     constexpr bool U = ROWS && sizeof(typename Codec<Tag>::raw) == 2;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
-    if constexpr (BC == BC_HALFWAY) {
-        // Without solids: interior rows and edge rows each with a collision
-        // and store of their own, compiled apart from each other and from
-        // the general path below (fewer live scalar registers in the
-        // interior rows' code).  Measured against one shared path, 256^3,
-        // GLUPS (FP16 / FP32): lid-driven cavity 73.0 -> 75.9 / 40.5 -> 40.5,
-        // half-way z walls 73.2 -> 76.4 / 40.1 -> 40.4; together with the
-        // uniform segment bases, reference y-walls in FP16 79.6 -> 82.9.
+    if constexpr (BC == BC_HALFWAY && sizeof(typename Codec<Tag>::raw) == 2) {
+        // 16-bit storage without solids: interior rows and edge rows each
+        // with a collision and store of their own, compiled apart from each
+        // other and from the general path below (fewer live scalar registers
+        // in the interior rows' code).  Measured against one shared path,
+        // 256^3 FP16, GLUPS: lid-driven cavity 73.0 -> 75.9, half-way z walls
+        // 73.2 -> 76.4; with the uniform segment bases, reference y-walls
+        // 79.6 -> 82.9.  (FP32 gained nothing at 256^3 and lost 11% on the
+        // latency-bound 64^3 cavity: 4- and 8-byte storage keep one path.)
         if (!p.solid) {
             if (!halfway_edge_row(p, x, y)) {
                 pull_halfway_fast<Tag, false, U>(in, p, x, y, z, d);
