@@ -6,8 +6,9 @@
 // :148-206, calibrate_ranges :208-253, run_lbm :283-486.  Every lattice
 // operation is a libd3q19 call on the default HIP stream: the trajectory, the
 // FP64 calibration run and reference_step run in the engine's kernels (FP64
-// storage, or D3Q19_ME16 codes); only the frame sums, the witness hash and the
-// capture files are host work.
+// storage; a quantized trajectory in D3Q19_ME16_REF codes, the variant's own
+// FP64 step, or on request D3Q19_ME16); only the frame sums, the witness hash
+// and the capture files are host work.
 #include "bit_physics/lbm_d3q19_me/lbm_me.hpp"
 
 #include <algorithm>
@@ -248,7 +249,12 @@ QuantRanges calibrate_ranges(const LbmConfig& cfg, const MomentBasis& basis) {
 }
 
 LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest) {
+    return run_lbm(cfg, capture_manifest, QuantArithmetic::reference_fp64);
+}
+
+LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest, QuantArithmetic arithmetic) {
     validate(cfg);
+    const int32_t quantized_storage = arithmetic == QuantArithmetic::reference_fp64 ? D3Q19_ME16_REF : D3Q19_ME16;
     const auto t0 = std::chrono::steady_clock::now();
     const std::string start = utc_now();
     const size_t n = ncell(cfg);
@@ -260,7 +266,7 @@ LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_man
     // One trajectory; frames (0, every capture_interval, the last step) when
     // out is set.  Returns the SHA-256 of the final FP64 populations.
     auto trajectory = [&](LbmResult* out) -> std::string {
-        Channel lat(cfg, cfg.quantize ? D3Q19_ME16 : D3Q19_F64);
+        Channel lat(cfg, cfg.quantize ? quantized_storage : D3Q19_F64);
         if (cfg.quantize) check(d3q19_set_moment_ranges(lat.h, ranges.lo.data(), ranges.hi.data()), "set_moment_ranges");
         lat.load(f0);  // ME16: the state starts as the codes of the rest state
         auto snapshot = [&](uint32_t s) {

@@ -12,6 +12,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <vector>
 
 #include "../../include/d3q19.h"
 #include "d3q19_host.hpp"
@@ -46,7 +47,8 @@ size_t storage_bytes(int s) {
         case D3Q19_F16:
         case D3Q19_BF16:
         case D3Q19_BF16SR:
-        case D3Q19_ME16: return 2;
+        case D3Q19_ME16:
+        case D3Q19_ME16_REF: return 2;
     }
     return 0;
 }
@@ -82,8 +84,13 @@ struct d3q19_lattice {
     void* uscratch = nullptr;  // read-back velocity (3, ncell) in the compute type (enstrophy)
     bool has_state = false;
     bool local = false;  // buffer cur holds pre-collision f^n in place
-    // D3Q19_ME16: buf[0] holds the moment codes, buf[1] the FP32 work lattice
+    // D3Q19_ME16: buf[0] holds the moment codes, buf[1] the FP32 work lattice.
+    // D3Q19_ME16_REF: buf[0] the codes (moment-major, k * ncell + cell), buf[1]
+    // and ref_b two FP64 population arrays in the reference layout.
     bool me = false;
+    bool me_ref = false;
+    void* ref_b = nullptr;
+    size_t codes_bytes = 0;
     bool me_ranges = false;
     double me_lo[Q] = {}, me_hi[Q] = {};
     double* mr_partials = nullptr;  // moment-range partials and result
@@ -438,7 +445,9 @@ int launch_diag(d3q19_lattice* L) {
         return fail(D3Q19_EINVAL, "bad storage type");                          \
     }()
 
-bool is_ct_double(const d3q19_lattice* L) { return L->cfg.storage == D3Q19_F64; }
+bool is_ct_double(const d3q19_lattice* L) {
+    return L->cfg.storage == D3Q19_F64 || L->cfg.storage == D3Q19_ME16_REF;
+}
 size_t ct_size(const d3q19_lattice* L) { return is_ct_double(L) ? 8 : 4; }
 
 int stage_to_compute(d3q19_lattice* L, const void* src, int32_t dt, int32_t on_device, size_t n,
@@ -629,6 +638,158 @@ int launch_health(d3q19_lattice* L) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// D3Q19_ME16_REF: the moment codes stepped in the moment-encoded variant's own
+// FP64 arithmetic (packages/lattice-boltzmann-d3q19-frontier-moment-encoded,
+// src/lbm_me.cpp:386-409: decode, collide, stream + bounce, encode).  A step
+// is two kernels: decode + collide into the FP64 post-collision array ref_b,
+// then pull-stream + y-walls + encode back into the codes.  buf[1] (ref_a)
+// holds decoded populations for read-back and the loads' staging.
+// Boundaries: periodic or the reference's y-walls.
+// ---------------------------------------------------------------------------
+extern "C" int d3q19_destroy(d3q19_lattice* L);
+
+namespace {
+
+MeRef make_me_ref(const d3q19_lattice* L) {
+    MeRef r;
+    for (int k = 0; k < Q; ++k) {
+        r.lo[k] = L->me_lo[k];
+        r.hi[k] = L->me_hi[k];
+    }
+    return r;
+}
+
+double* ref_a(const d3q19_lattice* L) { return (double*)L->buf[1]; }
+double* ref_b(const d3q19_lattice* L) { return (double*)L->ref_b; }
+
+int create_me_ref(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
+    if (cfg->boundary != D3Q19_BC_PERIODIC && cfg->boundary != D3Q19_BC_YWALLS)
+        return fail(D3Q19_EINVAL, "D3Q19_ME16_REF runs the reference's step: periodic or y-wall boundaries");
+    d3q19_lattice* L = new d3q19_lattice();
+    L->cfg = *cfg;
+    L->ncell = (uint32_t)((uint64_t)cfg->nx * cfg->ny * cfg->nz);
+    L->me = true;
+    L->me_ref = true;
+    L->rows = false;
+    L->esize = 2;
+    L->nzp = (uint32_t)cfg->nz;
+    L->dir_bytes = (size_t)L->ncell * 2;
+    L->row_bytes = (size_t)cfg->nz * 2;
+    L->codes_bytes = (size_t)Q * L->ncell * 2;
+    L->lat_bytes = (size_t)Q * L->ncell * 8;
+    L->stream = (hipStream_t)stream;
+    const size_t n3 = 3 * (size_t)L->ncell;
+    if (hipMalloc(&L->buf[0], L->codes_bytes) != hipSuccess || hipMalloc(&L->buf[1], L->lat_bytes) != hipSuccess ||
+        hipMalloc(&L->ref_b, L->lat_bytes) != hipSuccess || hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {
+        d3q19_destroy(L);
+        return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
+    }
+    L->bytes = L->codes_bytes + 2 * L->lat_bytes + 64;
+    if (cfg->force_mode == D3Q19_FORCE_UNIFORM) {
+        // the reference's bgk_step takes a per-cell force array
+        std::vector<double> fh(n3);
+        for (size_t a = 0; a < 3; ++a)
+            for (size_t c = 0; c < L->ncell; ++c) fh[a * L->ncell + c] = cfg->force[a];
+        if (hipMalloc(&L->force, n3 * sizeof(double)) != hipSuccess ||
+            hipMemcpy(L->force, fh.data(), n3 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+            d3q19_destroy(L);
+            return fail(D3Q19_ENOMEM, "force array allocation failed");
+        }
+        L->bytes += n3 * sizeof(double);
+    }
+    *out = L;
+    return D3Q19_OK;
+}
+
+int me_ref_decode(d3q19_lattice* L) {
+    k_me_ref_decode<<<nblocks(L->ncell), BLOCK, 0, L->stream>>>((const uint16_t*)L->buf[0], ref_a(L), make_me_ref(L),
+                                                               L->ncell);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+int me_ref_encode(d3q19_lattice* L, const double* f) {
+    k_me_ref_encode<<<nblocks(L->ncell), BLOCK, 0, L->stream>>>(f, (uint16_t*)L->buf[0], make_me_ref(L), L->ncell);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
+// f (19, ncell) of dtype dt, host or device -> FP64 array A -> codes.
+int me_ref_set_populations(d3q19_lattice* L, const void* f, int32_t dt, int32_t on_device) {
+    if (!L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16_REF lattice needs d3q19_set_moment_ranges first");
+    if (int e = stage_in(f, dt, on_device, (size_t)Q * L->ncell, ref_a(L), true, L->stream)) return e;
+    if (int e = me_ref_encode(L, ref_a(L))) return e;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->has_state = true;
+    L->local = true;
+    L->steps = 0;
+    return D3Q19_OK;
+}
+
+// feq_field(rho, u) in FP64 (rho may be NULL: rho = 1), then its codes.
+int me_ref_set_equilibrium(d3q19_lattice* L, const void* rho, const void* u, int32_t dt, int32_t on_device) {
+    if (!L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16_REF lattice needs d3q19_set_moment_ranges first");
+    const size_t nc = L->ncell;
+    double* ru = ref_b(L);  // rho (nc) then u (3 nc), FP64
+    if (rho)
+        if (int e = stage_in(rho, dt, on_device, nc, ru, true, L->stream)) return e;
+    if (int e = stage_in(u, dt, on_device, 3 * nc, ru + nc, true, L->stream)) return e;
+    k_fn_feq<double><<<nblocks(nc), BLOCK, 0, L->stream>>>(rho ? ru : nullptr, ru + nc, ref_a(L), (uint32_t)nc);
+    HIPCHK(hipGetLastError());
+    if (int e = me_ref_encode(L, ref_a(L))) return e;
+    HIPCHK(hipStreamSynchronize(L->stream));
+    L->has_state = true;
+    L->local = true;
+    L->steps = 0;
+    return D3Q19_OK;
+}
+
+int me_ref_step(d3q19_lattice* L, int32_t n) {
+    const uint32_t nc = L->ncell;
+    const double* force = L->cfg.force_mode == D3Q19_FORCE_NONE ? nullptr : (const double*)L->force;
+    const MeRef r = make_me_ref(L);
+    MeRefWalls w;
+    w.on = L->cfg.boundary == D3Q19_BC_YWALLS;
+    for (int a = 0; a < 3; ++a) {
+        w.top[a] = L->cfg.wall_velocity_top[a];
+        w.bot[a] = L->cfg.wall_velocity_bottom[a];
+    }
+    uint16_t* codes = (uint16_t*)L->buf[0];
+    for (int32_t s = 0; s < n; ++s) {
+        k_me_ref_decode_collide<<<nblocks(nc), BLOCK, 0, L->stream>>>(codes, ref_b(L), force, r, nc, L->cfg.tau);
+        HIPCHK(hipGetLastError());
+        k_me_ref_stream_encode<<<nblocks(nc), BLOCK, 0, L->stream>>>(ref_b(L), codes, r, w, L->cfg.nx, L->cfg.ny,
+                                                                     L->cfg.nz);
+        HIPCHK(hipGetLastError());
+        L->steps++;
+    }
+    return D3Q19_OK;
+}
+
+// Decoded f^n and/or density_field / macroscopic_velocity(f^n, force) of it.
+int me_ref_read(d3q19_lattice* L, void* f, void* rho, void* u, int32_t dt, int32_t on_device) {
+    const size_t nc = L->ncell;
+    if (int e = me_ref_decode(L)) return e;
+    if (f)
+        if (int e = stage_out(ref_a(L), true, (size_t)Q * nc, f, dt, on_device, L->stream)) return e;
+    if (rho || u) {
+        const double* force = L->cfg.force_mode == D3Q19_FORCE_NONE ? nullptr : (const double*)L->force;
+        double* ru = ref_b(L);
+        k_fn_moments<double><<<nblocks(nc), BLOCK, 0, L->stream>>>(ref_a(L), force, ru, nullptr, ru + nc,
+                                                                   (uint32_t)nc);
+        HIPCHK(hipGetLastError());
+        if (rho)
+            if (int e = stage_out(ru, true, nc, rho, dt, on_device, L->stream)) return e;
+        if (u)
+            if (int e = stage_out(ru + nc, true, 3 * nc, u, dt, on_device, L->stream)) return e;
+    }
+    HIPCHK(hipStreamSynchronize(L->stream));
+    return D3Q19_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // Allocation-free read-back of the moments to host memory.  The stateful
 // d3q19_read_moments stages host outputs through a device buffer it
 // allocates per call; this variant stages them through `scratch`, a
@@ -709,6 +870,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     if (cfg->boundary == D3Q19_BC_HALFWAY && (cfg->wall_faces & ~63u))
         return fail(D3Q19_EINVAL, "wall_faces has bits beyond the six faces");
     if (hipSetDevice(cfg->device) != hipSuccess) return fail(D3Q19_EHIP, "hipSetDevice failed");
+    if (cfg->storage == D3Q19_ME16_REF) return create_me_ref(cfg, stream, out);
 
     d3q19_lattice* L = new d3q19_lattice();
     L->cfg = *cfg;
@@ -746,6 +908,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
         return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
     }
     L->bytes = buf0_bytes + lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
+    if (me) L->codes_bytes = buf0_bytes;
     // Row padding (z >= nz of a padded row) is never written by a kernel:
     // zero it once, so that the raw state (read_state, checkpoints) is a
     // function of the lattice's content alone.
@@ -762,8 +925,8 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
 int d3q19_destroy(d3q19_lattice* L) {
     if (!L) return D3Q19_OK;
     (void)hipStreamSynchronize(L->stream);
-    void* ps[] = {L->buf[0], L->buf[1], L->force,       L->brink, L->solid, L->partials,
-                  L->dout,   L->uscratch, L->mr_partials, L->mr_out};
+    void* ps[] = {L->buf[0], L->buf[1],   L->force,       L->brink,  L->solid, L->partials,
+                  L->dout,   L->uscratch, L->mr_partials, L->mr_out, L->ref_b};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete L;
@@ -781,6 +944,7 @@ int d3q19_set_populations(d3q19_lattice* L, const void* f, int32_t dt, int32_t o
     if (int e = check_lat(L)) return e;
     if (int e = check_dtype(dt)) return e;
     if (!f) return fail(D3Q19_EINVAL, "null populations");
+    if (L->me_ref) return me_ref_set_populations(L, f, dt, on_device);
     const size_t n = (size_t)Q * L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf tmp;
@@ -805,6 +969,7 @@ int d3q19_set_equilibrium(d3q19_lattice* L, const void* rho, const void* u, int3
     if (int e = check_lat(L)) return e;
     if (int e = check_dtype(dt)) return e;
     if (!u) return fail(D3Q19_EINVAL, "null velocity");
+    if (L->me_ref) return me_ref_set_equilibrium(L, rho, u, dt, on_device);
     const size_t nc = L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf drho, du, df;
@@ -857,6 +1022,7 @@ int d3q19_set_force_field(d3q19_lattice* L, const void* force, int32_t dt, int32
 
 int d3q19_set_brinkman(d3q19_lattice* L, const void* s, int32_t dt, int32_t on_device) {
     if (int e = check_lat(L)) return e;
+    if (L->me_ref) return fail(D3Q19_EINVAL, "D3Q19_ME16_REF runs the reference's step, which has no Brinkman term");
     if (!s) {
         if (L->brink) {
             HIPCHK(hipStreamSynchronize(L->stream));
@@ -903,6 +1069,7 @@ int d3q19_step(d3q19_lattice* L, int32_t n) {
     if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
         return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
     if (n == 0) return D3Q19_OK;
+    if (L->me_ref) return me_ref_step(L, n);
     return D3Q19_DISPATCH(L, launch_step, L, n, nullptr, nullptr);
 }
 
@@ -912,6 +1079,7 @@ int d3q19_step_moments(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     if (n < 1) return fail(D3Q19_EINVAL, "step_moments needs n >= 1");
     if (L->cfg.force_mode == D3Q19_FORCE_FIELD && !L->force)
         return fail(D3Q19_ESTATE, "FORCE_FIELD lattice needs d3q19_set_force_field first");
+    if (L->me_ref) return fail(D3Q19_EINVAL, "step_moments is not available for D3Q19_ME16_REF lattices");
     return D3Q19_DISPATCH(L, launch_step, L, n, rho, u);
 }
 
@@ -920,6 +1088,7 @@ int d3q19_read_populations(d3q19_lattice* L, void* f, int32_t dt, int32_t on_dev
     if (int e = check_dtype(dt)) return e;
     if (!f) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (L->me_ref) return me_ref_read(L, f, nullptr, nullptr, dt, on_device);
     const size_t n = (size_t)Q * L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf tmp;
@@ -942,6 +1111,7 @@ int d3q19_read_moments(d3q19_lattice* L, void* rho, void* u, int32_t dt, int32_t
     if (int e = check_dtype(dt)) return e;
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
     if (!rho && !u) return D3Q19_OK;
+    if (L->me_ref) return me_ref_read(L, nullptr, rho, u, dt, on_device);
     const size_t nc = L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf trho, tu;
@@ -974,6 +1144,7 @@ int d3q19_read_moments_noalloc(d3q19_lattice* L, void* rho, void* u, int32_t dt)
     if (int e = check_dtype(dt)) return e;
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
     if (!rho && !u) return D3Q19_OK;
+    if (L->me_ref) return fail(D3Q19_EINVAL, "read_moments_noalloc is not available for D3Q19_ME16_REF lattices");
     const size_t nc = L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     const size_t need = 4 * nc * es;  // rho (ncell) then u (3, ncell)
@@ -1002,6 +1173,7 @@ int d3q19_diagnostics(d3q19_lattice* L, double out[5]) {
     if (int e = check_lat(L)) return e;
     if (!out) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (L->me_ref) return fail(D3Q19_EINVAL, "diagnostics is not available for D3Q19_ME16_REF lattices");
     int rc = D3Q19_DISPATCH(L, launch_diag, L);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(out, L->dout, NDIAG * sizeof(double), hipMemcpyDeviceToHost, L->stream));
@@ -1013,6 +1185,7 @@ int d3q19_enstrophy(d3q19_lattice* L, double* out) {
     if (int e = check_lat(L)) return e;
     if (!out) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (L->me_ref) return fail(D3Q19_EINVAL, "enstrophy is not available for D3Q19_ME16_REF lattices");
     // Central differences wrap around every axis: only a fully periodic
     // lattice (no wall face, no solid cell) has a well-defined value here.
     const bool periodic = L->cfg.boundary == D3Q19_BC_PERIODIC ||
@@ -1075,6 +1248,7 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
     if (int e = check_lat(L)) return e;
     if (!lo || !hi) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (L->me_ref) return fail(D3Q19_EINVAL, "moment_range is not available for D3Q19_ME16_REF lattices");
     if (!L->mr_partials || !L->mr_out) {
         // one slot pair per block of the population grid (<= the partials of the diagnostics)
         const size_t nb = L->rows ? (size_t)L->cfg.nx * L->cfg.ny * ((L->cfg.nz + 63) / 64) : nblocks(L->ncell);
@@ -1103,7 +1277,7 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
 
 size_t d3q19_state_bytes(const d3q19_lattice* L) {
     if (!L) return 0;
-    return L->me ? L->lat_bytes / 2 : L->lat_bytes;
+    return L->me ? L->codes_bytes : L->lat_bytes;
 }
 
 int d3q19_read_state(d3q19_lattice* L, void* dst, int32_t* local, int64_t* steps) {
@@ -1143,6 +1317,7 @@ int d3q19_health(d3q19_lattice* L, double out[4]) {
     if (int e = check_lat(L)) return e;
     if (!out) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
+    if (L->me_ref) return fail(D3Q19_EINVAL, "health is not available for D3Q19_ME16_REF lattices");
     if (int rc = D3Q19_DISPATCH(L, launch_health, L)) return rc;
     HIPCHK(hipMemcpyAsync(out, L->dout, NHEALTH * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));

@@ -416,8 +416,11 @@ __device__ __forceinline__ T ref_feq_point(int i, T r, T ux, T uy, T uz, T u_sq)
     return wgt<T>(i) * r * (T(1) + cu / cs2 + (cu * cu) / (T(2) * cs2 * cs2) - u_sq / (T(2) * cs2));
 }
 
-// reference/bgk.py:72-110 collision of one cell.
-template <class T>
+// reference/bgk.py:72-110 collision of one cell.  POINT: the equilibrium in
+// the point function's division form (ref_feq_point), which is the
+// moment-encoded variant's collision shader's (shaders/lbm_collide.comp:58-65;
+// its u_eq and Guo term are bgk.py's with the force (force_x, 0, 0)).
+template <class T, bool POINT = false>
 __device__ __forceinline__ void ref_collide(T fc[Q], T tau, bool has_force, T fx, T fy, T fz) {
 #pragma clang fp contract(off)
     const RefConst<T> k = ref_const<T>();
@@ -434,7 +437,7 @@ __device__ __forceinline__ void ref_collide(T fc[Q], T tau, bool has_force, T fx
     T pref = T(1) - T(0.5) / tau;
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
-        T feq = ref_feq(i, r, ux, uy, uz, u_sq, k);
+        T feq = POINT ? ref_feq_point(i, r, ux, uy, uz, u_sq) : ref_feq(i, r, ux, uy, uz, u_sq, k);
         T fp = fc[i] - (fc[i] - feq) / tau;
         if (has_force) {
             T c0 = T(CX[i]), c1 = T(CY[i]), c2 = T(CZ[i]);

@@ -27,6 +27,7 @@
 #pragma once
 
 #include "d3q19_device.hpp"
+#include "d3q19_kernels.hpp"
 
 namespace d3q19 {
 
@@ -229,6 +230,142 @@ static __global__ void __launch_bounds__(256)
         me_load(codes, p, mc, x, y, z, d);
     }
     store_local<TF32>(work, p, x, y, z, d);
+}
+
+// ---------------------------------------------------------------------------
+// The moment-encoded variant's step in its own FP64 arithmetic (storage
+// D3Q19_ME16_REF), on codes in the variant's moment-major order (k * ncell +
+// cell) and FP64 populations in the reference layout (19, ncell):
+//   decode  m_k = lo_k + code_k (hi_k - lo_k) / 65535,  f_i = sum_k Minv[i][k] m_k
+//           (shaders/lbm_decode.comp:33-43),
+//   collide BGK + Guo with the division-form equilibrium (lbm_collide.comp),
+//   stream  the pull f_i(x) = f_i^post(x - c_i), then the half-way y-walls
+//           (lbm_stream_bounce.comp; moving walls take apply_bounce_back_y_walls'
+//           momentum term, zero for the variant's resting walls),
+//   encode  m_k = sum_i M[k][i] f_i,  t = clamp((m_k - lo_k) / (hi_k - lo_k), 0, 1),
+//           code_k = round(t 65535), halves away from zero
+//           (shaders/lbm_encode.comp:37-51),
+// every sum over all 19 terms in index order, contraction off.  M^-1 holds
+// quarters, so its products are exact.  One step is two kernels: decode +
+// collide (codes -> FP64 post-collision populations) and stream + walls +
+// encode (-> codes).
+// ---------------------------------------------------------------------------
+struct MeRef {
+    double lo[Q], hi[Q];
+};
+
+__device__ __forceinline__ void me_ref_decode_cell(const uint16_t* __restrict__ codes, const MeRef& r,
+                                                   uint32_t ncell, uint32_t cell, double f[Q]) {
+#pragma clang fp contract(off)
+    double m[Q];
+#pragma unroll
+    for (int k = 0; k < Q; ++k)
+        m[k] = r.lo[k] + (double)codes[(size_t)k * ncell + cell] * (r.hi[k] - r.lo[k]) / 65535.0;
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < Q; ++k) s += ((double)ME_MINV4[i][k] * 0.25) * m[k];
+        f[i] = s;
+    }
+}
+
+__device__ __forceinline__ void me_ref_encode_cell(const double f[Q], const MeRef& r, uint32_t ncell,
+                                                   uint32_t cell, uint16_t* __restrict__ codes) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        double mk = 0.0;
+#pragma unroll
+        for (int i = 0; i < Q; ++i) mk += (double)me_mono(k, i) * f[i];
+        double t = (mk - r.lo[k]) / (r.hi[k] - r.lo[k]);
+        t = t > 0.0 ? (t > 1.0 ? 1.0 : t) : 0.0;  // a NaN moment takes code 0
+        codes[(size_t)k * ncell + cell] = (uint16_t)round(t * 65535.0);
+    }
+}
+
+static __global__ void __launch_bounds__(256)
+    k_me_ref_decode(const uint16_t* __restrict__ codes, double* __restrict__ f, MeRef r, uint32_t ncell) {
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= ncell) return;
+    double fc[Q];
+    me_ref_decode_cell(codes, r, ncell, cell, fc);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) f[(size_t)i * ncell + cell] = fc[i];
+}
+
+static __global__ void __launch_bounds__(256)
+    k_me_ref_encode(const double* __restrict__ f, uint16_t* __restrict__ codes, MeRef r, uint32_t ncell) {
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= ncell) return;
+    double fc[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) fc[i] = f[(size_t)i * ncell + cell];
+    me_ref_encode_cell(fc, r, ncell, cell, codes);
+}
+
+// decode + collide: post = collide(decode(codes)); force (3, ncell) or null.
+static __global__ void __launch_bounds__(256)
+    k_me_ref_decode_collide(const uint16_t* __restrict__ codes, double* __restrict__ post,
+                            const double* __restrict__ force, MeRef r, uint32_t ncell, double tau) {
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= ncell) return;
+    double fc[Q];
+    me_ref_decode_cell(codes, r, ncell, cell, fc);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (force) {
+        fx = force[cell];
+        fy = force[(size_t)ncell + cell];
+        fz = force[2 * (size_t)ncell + cell];
+    }
+    ref_collide<double, true>(fc, tau, force != nullptr, fx, fy, fz);
+#pragma unroll
+    for (int i = 0; i < Q; ++i) post[(size_t)i * ncell + cell] = fc[i];
+}
+
+// stream + (y-walls) + encode: codes = encode(walls(pull(post))).
+struct MeRefWalls {
+    int on;
+    double top[3], bot[3];
+};
+
+static __global__ void __launch_bounds__(256)
+    k_me_ref_stream_encode(const double* __restrict__ post, uint16_t* __restrict__ codes, MeRef r, MeRefWalls w,
+                           uint32_t nx, uint32_t ny, uint32_t nz) {
+#pragma clang fp contract(off)
+    const uint32_t ncell = nx * ny * nz;
+    const uint32_t cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= ncell) return;
+    uint32_t x, y, z;
+    cell_coords(cell, ny, nz, x, y, z);
+    double g[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        const uint32_t sx = (x + nx - CX[i]) % nx, sy = (y + ny - CY[i]) % ny, sz = (z + nz - CZ[i]) % nz;
+        g[i] = post[(size_t)i * ncell + (sx * ny + sy) * nz + sz];
+    }
+    double o[Q];
+#pragma unroll
+    for (int i = 0; i < Q; ++i) o[i] = g[i];
+    if (w.on && (y == 0 || y + 1 == ny)) {
+        // apply_bounce_back_y_walls (reference/bgk.py:145-216) on the pulled values
+        const double cs2 = 1.0 / 3.0;
+        double rw = 0.0;
+#pragma unroll
+        for (int d = 0; d < Q; ++d) rw = rw + g[d];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            if (CY[i] > 0 && y == 0) {
+                const double cdu = (double)CX[i] * w.bot[0] + (double)CY[i] * w.bot[1] + (double)CZ[i] * w.bot[2];
+                o[i] = g[OPP[i]] + -2.0 * wgt<double>(i) * rw * cdu / cs2;
+            }
+            if (CY[i] < 0 && y + 1 == ny) {
+                const double cdu = (double)CX[i] * w.top[0] + (double)CY[i] * w.top[1] + (double)CZ[i] * w.top[2];
+                o[i] = g[OPP[i]] + -2.0 * wgt<double>(i) * rw * cdu / cs2;
+            }
+        }
+    }
+    me_ref_encode_cell(o, r, ncell, cell, codes);
 }
 
 // Per-block minimum and maximum of the 19 moments m = M f (FP64) of the

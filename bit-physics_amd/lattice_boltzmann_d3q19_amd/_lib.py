@@ -12,12 +12,12 @@ from pathlib import Path
 LIB_PATH = Path(__file__).resolve().parent.parent / "lib" / "libd3q19.so"
 
 D3Q19_OK, D3Q19_EINVAL, D3Q19_ENOMEM, D3Q19_EHIP, D3Q19_ESTATE = 0, 1, 2, 3, 4
-D3Q19_F64, D3Q19_F32, D3Q19_F16, D3Q19_BF16, D3Q19_ME16, D3Q19_BF16SR = 0, 1, 2, 3, 4, 5
+D3Q19_F64, D3Q19_F32, D3Q19_F16, D3Q19_BF16, D3Q19_ME16, D3Q19_BF16SR, D3Q19_ME16_REF = 0, 1, 2, 3, 4, 5, 6
 BC_PERIODIC, BC_YWALLS, BC_HALFWAY = 0, 1, 2
 FORCE_NONE, FORCE_UNIFORM, FORCE_FIELD = 0, 1, 2
 
 STORAGE = {"f64": D3Q19_F64, "f32": D3Q19_F32, "f16": D3Q19_F16, "bf16": D3Q19_BF16, "bf16sr": D3Q19_BF16SR,
-           "me16": D3Q19_ME16}
+           "me16": D3Q19_ME16, "me16ref": D3Q19_ME16_REF}
 BOUNDARY = {"periodic": BC_PERIODIC, "ywalls": BC_YWALLS, "halfway": BC_HALFWAY}
 FACES = {"-x": 0, "+x": 1, "-y": 2, "+y": 3, "-z": 4, "+z": 5}
 

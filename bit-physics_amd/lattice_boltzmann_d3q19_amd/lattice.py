@@ -59,9 +59,12 @@ class Lattice:
     ----------
     shape : (Nx, Ny, Nz)
     tau : BGK relaxation time (> 0.5)
-    storage : "f32" (default) | "f64" | "f16" | "bf16" | "bf16sr" | "me16" -- population
-        storage; 16-bit formats store f_i - w_i ("bf16sr": stochastically
-        rounded) or 16-bit moment codes ("me16"); collision stays FP32.
+    storage : "f32" (default) | "f64" | "f16" | "bf16" | "bf16sr" | "me16" | "me16ref" --
+        population storage; 16-bit formats store f_i - w_i ("bf16sr":
+        stochastically rounded) or 16-bit moment codes ("me16"); collision
+        stays FP32.  "me16ref": the same moment codes stepped in the
+        moment-encoded reference variant's FP64 arithmetic (periodic or
+        y-walls; no reductions).
     boundary : "periodic" | "ywalls" (the reference's apply_bounce_back_y_walls)
         | "halfway" (textbook half-way bounce-back on `wall_faces`, solids).
     wall_velocity_top / wall_velocity_bottom : ywalls mode wall velocities.

@@ -64,15 +64,18 @@ def calibrate_moment_ranges(shape, tau, steps, *, margin=RANGE_MARGIN, warmup_st
 
 
 def moment_encoded_lattice(shape, tau, steps, *, ranges=None, margin=RANGE_MARGIN, warmup_steps=WARMUP_STEPS,
-                           populations=None, rho=None, u=None, prepare=None, **lattice_kw):
+                           populations=None, rho=None, u=None, prepare=None, reference_arithmetic=False,
+                           **lattice_kw):
     """An "me16" lattice with calibrated (or given) ranges and the initial
     state loaded; ``steps`` is the horizon the ranges must cover and
     ``prepare(lattice)`` is applied to the calibration and the "me16"
-    lattice alike."""
+    lattice alike.  ``reference_arithmetic=True`` gives an "me16ref"
+    lattice: the same codes, stepped in the reference variant's FP64
+    arithmetic (decode, bgk_step, y-walls, encode), periodic or y-walls."""
     if ranges is None:
         ranges = calibrate_moment_ranges(shape, tau, steps, margin=margin, warmup_steps=warmup_steps,
                                          populations=populations, rho=rho, u=u, prepare=prepare, **lattice_kw)
-    lat = Lattice(shape, tau, storage="me16", **lattice_kw)
+    lat = Lattice(shape, tau, storage="me16ref" if reference_arithmetic else "me16", **lattice_kw)
     lat.set_moment_ranges(*ranges)
     if prepare is not None:
         prepare(lat)

@@ -151,7 +151,8 @@ napi_value Construct(napi_env env, napi_callback_info info) {
         else if (s == "bf16") cfg.storage = D3Q19_BF16;
         else if (s == "bf16sr") cfg.storage = D3Q19_BF16SR;
         else if (s == "me16") cfg.storage = D3Q19_ME16;
-        else { napi_throw_range_error(env, nullptr, "storage must be f64|f32|f16|bf16|bf16sr|me16"); return nullptr; }
+        else if (s == "me16ref") cfg.storage = D3Q19_ME16_REF;
+        else { napi_throw_range_error(env, nullptr, "storage must be f64|f32|f16|bf16|bf16sr|me16|me16ref"); return nullptr; }
     }
     if (get_prop_string(env, argv[0], "boundary", &s)) {
         if (s == "periodic") cfg.boundary = D3Q19_BC_PERIODIC;

@@ -2,7 +2,7 @@
 // Populations are (19, Nx, Ny, Nz) z-fastest in the reference's lexicographic
 // velocity order; vector fields are (3, Nx, Ny, Nz).
 
-export type Storage = 'f64' | 'f32' | 'f16' | 'bf16' | 'bf16sr' | 'me16';
+export type Storage = 'f64' | 'f32' | 'f16' | 'bf16' | 'bf16sr' | 'me16' | 'me16ref';
 export type Boundary = 'periodic' | 'ywalls' | 'halfway';
 export type Face = '-x' | '+x' | '-y' | '+y' | '-z' | '+z';
 export type Vec3 = [number, number, number];
@@ -51,7 +51,7 @@ export declare class Lattice {
   readMoments(): Moments;
   diagnostics(): Diagnostics;
   stepCount(): number;
-  // 'me16' storage: quantization ranges of the 19 moments of f, before the state is loaded
+  // 'me16' / 'me16ref' storage: quantization ranges of the 19 moments of f, before the state is loaded
   setMomentRanges(lo: Float64Array, hi: Float64Array): void;
   // minimum and maximum over all cells of the 19 moments of the current state
   momentRange(): { lo: Float64Array; hi: Float64Array };

@@ -112,4 +112,14 @@ std::vector<double> initial_rest_state(const LbmConfig& cfg);
 // named after the manifest's stem.
 LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest);
 
+// Not in the reference's interface: the arithmetic of a quantized run.
+// reference_fp64 (what run_lbm above uses) steps the codes as the reference
+// variant's shaders do, decoding to FP64 populations, colliding (division-form
+// equilibrium), streaming with the y-wall bounce-back and encoding the result
+// (D3Q19_ME16_REF); engine_fp32 uses the engine's two-pass FP32 moment-coded
+// step (D3Q19_ME16: deviation moments, FP32 collision), faster and within the
+// lbm-quantized tolerance of the FP64 one.
+enum class QuantArithmetic { reference_fp64, engine_fp32 };
+LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest, QuantArithmetic arithmetic);
+
 }  // namespace bit_physics::lbm_d3q19_me

@@ -42,7 +42,21 @@ typedef enum d3q19_status {
 
 /* Storage D3Q19_BF16SR is D3Q19_BF16 with stochastic rounding of every store
  * (unbiased; the rounding bits are a hash of cell and step, so runs repeat). */
-enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3, D3Q19_ME16 = 4, D3Q19_BF16SR = 5 };
+/* Storage D3Q19_ME16_REF holds the same 16-bit moment codes as D3Q19_ME16,
+ * moment-major (k * ncell + cell), and steps them in the moment-encoded
+ * variant's own FP64 arithmetic (packages/lattice-boltzmann-d3q19-frontier-
+ * moment-encoded/src/lbm_me.cpp:386-409 and its shaders): decode to full FP64
+ * populations (lbm_decode.comp), BGK + Guo collision with the division-form
+ * equilibrium (lbm_collide.comp), pull stream and (y-walls) half-way
+ * bounce-back (lbm_stream_bounce.comp; moving walls take
+ * apply_bounce_back_y_walls' momentum term), encode (lbm_encode.comp).
+ * Periodic or y-wall boundaries; set_populations, set_equilibrium, step,
+ * read_populations, read_moments, set_force_field and the raw state; the
+ * reductions (diagnostics, health, enstrophy, moment_range), step_moments
+ * and read_moments_noalloc return D3Q19_EINVAL.  Device memory: 2 B codes + 2 x 152 B
+ * FP64 populations per cell. */
+enum { D3Q19_F64 = 0, D3Q19_F32 = 1, D3Q19_F16 = 2, D3Q19_BF16 = 3, D3Q19_ME16 = 4, D3Q19_BF16SR = 5,
+       D3Q19_ME16_REF = 6 };
 
 enum {
     D3Q19_BC_PERIODIC = 0, /* periodic on all six faces (NP/reference/bgk.py:115-126) */

@@ -71,7 +71,7 @@ def _load():
     return _lib
 
 
-PLANTS = ("KE_NO_HALF", "KE_UNIT_RHO", "KE_NO_GUO", "SOLID_SAME_DIR", "SOLID_WRONG_SIDE")
+PLANTS = ("KE_NO_HALF", "KE_UNIT_RHO", "KE_NO_GUO", "SOLID_SAME_DIR", "SOLID_WRONG_SIDE", "ME_FIELD_FEQ")
 
 
 @contextlib.contextmanager
@@ -104,6 +104,7 @@ def _bind(lib):
         "orc_momentum_field": (None, [i64, P, P]),
         "orc_macroscopic_velocity": (None, [i64, P, P, P]),
         "orc_collide": (None, [i64, P, P, f64, P]),
+        "orc_me_collide": (None, [i64, P, P, f64, P]),
         "orc_stream": (None, [i32, i32, i32, P, P]),
         "orc_bgk_step": (None, [i32, i32, i32, P, P, f64, P, P]),
         "orc_bounce_back_y_walls": (None, [i32, i32, i32, P, P, P, P]),
@@ -117,6 +118,7 @@ def _bind(lib):
         "orc_me_encode": (None, [i64, P, P, P, P]),
         "orc_me_decode": (None, [i64, P, P, P, P]),
         "orc_run_me": (None, [i32, i32, i32, P, f64, P, i32, P, P, i32, P, P, P, P]),
+        "orc_run_me_codes": (None, [i32, i32, i32, P, f64, P, i32, P, P, i32, P, P, P, P]),
         "orc_max_threads": (i32, []),
         "orc_set_threads": (None, [i32]),
     }
@@ -128,7 +130,9 @@ def _bind(lib):
 
 
 def _p(a):
-    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+    # data_as keeps a reference to the array: a temporary such as _f64(view)
+    # stays alive for as long as the pointer (the call's argument) does
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
 
 
 def _f64(a):
@@ -186,6 +190,16 @@ def collide(f, tau, force_lattice=None):
     F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3,) + f.shape[1:]))
     out = np.empty_like(f)
     _load().orc_collide(f[0].size, _p(f), _p(out), float(tau), _p(F))
+    return out
+
+
+def me_collide(f, tau, force_lattice=None):
+    """The moment-encoded variant's collision (ME/shaders/lbm_collide.comp):
+    collide() with the equilibrium in the division form of the point function."""
+    f = _f64(f)
+    F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3,) + f.shape[1:]))
+    out = np.empty_like(f)
+    _load().orc_me_collide(f[0].size, _p(f), _p(out), float(tau), _p(F))
     return out
 
 
@@ -304,17 +318,36 @@ def me_decode(codes, lo, hi):
 
 
 def run_me(f, tau, steps, lo, hi, force_lattice=None, ywalls=False,
-           wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0)):
-    """Quantized trajectory (ME/src/lbm_me.cpp:376-393): encode, then per step
-    decode, bgk_step, y-walls, encode; returns the decoded final state."""
-    f = _f64(f).copy()
-    nx, ny, nz = f.shape[1:]
+           wall_velocity_top=(0.0, 0.0, 0.0), wall_velocity_bottom=(0.0, 0.0, 0.0), return_codes=False,
+           codes=None):
+    """Quantized trajectory of the moment-encoded variant (ME/src/lbm_me.cpp:
+    376-409): encode f, then per step decode, collide (me_collide), stream,
+    y-walls, encode; returns the decoded final state (and, with
+    ``return_codes``, the final codes (19, Nx, Ny, Nz) uint16).
+
+    ``codes`` (uint16, the shape of f) continues a trajectory from its codes
+    instead of encoding f (f may then be None).  Decoding then re-encoding is
+    not the identity for a moment whose range is narrower than FP64 resolves
+    around its value, so a run restarted from decoded populations is another
+    trajectory than the one continued from its codes."""
+    if codes is None:
+        f = _f64(f).copy()
+        shape = f.shape
+    else:
+        shape = codes.shape
+        f = np.empty(shape, np.float64)
+    nx, ny, nz = shape[1:]
     F = None if force_lattice is None else _f64(np.broadcast_to(force_lattice, (3, nx, ny, nz)))
-    codes = np.empty(f.shape, np.uint16)
-    work = np.empty((2,) + f.shape, np.float64)
-    _load().orc_run_me(nx, ny, nz, _p(f), float(tau), _p(F), int(bool(ywalls)), _p(_f64(wall_velocity_top)),
-                       _p(_f64(wall_velocity_bottom)), int(steps), _p(_f64(lo)), _p(_f64(hi)), _p(codes), _p(work))
-    return f
+    work = np.empty((2,) + shape, np.float64)
+    args = (float(tau), _p(F), int(bool(ywalls)), _p(_f64(wall_velocity_top)), _p(_f64(wall_velocity_bottom)),
+            int(steps), _p(_f64(lo)), _p(_f64(hi)))
+    if codes is None:
+        c = np.empty(shape, np.uint16)
+        _load().orc_run_me(nx, ny, nz, _p(f), *args, _p(c), _p(work))
+    else:
+        c = np.array(codes, dtype=np.uint16, order="C", copy=True)
+        _load().orc_run_me_codes(nx, ny, nz, _p(c), *args, _p(f), _p(work))
+    return (f, c) if return_codes else f
 
 
 def me_moments(f):

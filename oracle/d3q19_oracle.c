@@ -44,6 +44,8 @@
  *   KE_NO_GUO         kinetic energy from the unshifted velocity m / rho
  *   SOLID_SAME_DIR    bounce from a solid source returns h_i instead of h_opp(i)
  *   SOLID_WRONG_SIDE  the solid test probes x + c_i instead of the source x - c_i
+ *   ME_FIELD_FEQ      the moment-encoded variant's collision uses the field-form
+ *                     equilibrium (reciprocal multiplies) instead of the division form
  * The default build defines none of them.
  */
 #include <math.h>
@@ -95,13 +97,15 @@ void orc_constants(int32_t* c_out /*19x3*/, double* w_out /*19*/, int32_t* opp_o
 }
 
 /* equilibrium.py:34-53 -- point form, division operands as written there. */
+static inline double feq_point_one(int i, double r, double ux, double uy, double uz, double u_sq) {
+    double cu = (double)CV[i][0] * ux + (double)CV[i][1] * uy + (double)CV[i][2] * uz;
+    return WT[i] * r * (1.0 + cu / CS2 + (cu * cu) / (2.0 * CS2 * CS2) - u_sq / (2.0 * CS2));
+}
+
 void orc_feq_point(double rho, double ux, double uy, double uz, double* out) {
     init_constants();
     double u_sq = ux * ux + uy * uy + uz * uz;
-    for (int i = 0; i < Q; ++i) {
-        double cu = CV[i][0] * ux + CV[i][1] * uy + CV[i][2] * uz;
-        out[i] = WT[i] * rho * (1.0 + cu / CS2 + (cu * cu) / (2.0 * CS2 * CS2) - u_sq / (2.0 * CS2));
-    }
+    for (int i = 0; i < Q; ++i) out[i] = feq_point_one(i, rho, ux, uy, uz, u_sq);
 }
 
 /* equilibrium.py:71-98 -- field form, reciprocal operands. */
@@ -199,8 +203,12 @@ void orc_macroscopic_velocity(int64_t ncell, const double* f, const double* forc
  * s in [0,1] the solid fraction and u_eq0 = m/rho the unforced velocity,
  * the usual penalised (Brinkman) drag.
  */
-static inline void collide_one(const double* fc, double* out, double tau, int has_force, double fx,
-                               double fy, double fz) {
+/* point_feq selects the equilibrium's form: 0 the field form of bgk.py
+ * (reciprocal multiplies, equilibrium.py:87-98), 1 the division form of the
+ * point function (equilibrium.py:34-53), which is the moment-encoded
+ * variant's collision shader's (ME/shaders/lbm_collide.comp:58-65). */
+static inline void collide_one_form(const double* fc, double* out, double tau, int has_force, double fx,
+                                    double fy, double fz, int point_feq) {
     double r, mx, my, mz;
     moments_one(fc, &r, &mx, &my, &mz);
     double rs = rho_floor(r);
@@ -213,7 +221,10 @@ static inline void collide_one(const double* fc, double* out, double tau, int ha
     double u_sq = ux * ux + uy * uy + uz * uz;
     double pref = 1.0 - 0.5 / tau;
     for (int i = 0; i < Q; ++i) {
-        double feq = feq_one(i, r, ux, uy, uz, u_sq);
+#ifdef ORC_PLANT_ME_FIELD_FEQ
+        point_feq = 0;
+#endif
+        double feq = point_feq ? feq_point_one(i, r, ux, uy, uz, u_sq) : feq_one(i, r, ux, uy, uz, u_sq);
         double fp = fc[i] - (fc[i] - feq) / tau;
         if (has_force) {
             double c0 = (double)CV[i][0], c1 = (double)CV[i][1], c2 = (double)CV[i][2];
@@ -226,6 +237,39 @@ static inline void collide_one(const double* fc, double* out, double tau, int ha
         }
         out[i] = fp;
     }
+}
+
+static inline void collide_one(const double* fc, double* out, double tau, int has_force, double fx,
+                               double fy, double fz) {
+    collide_one_form(fc, out, tau, has_force, fx, fy, fz, 0);
+}
+
+static void collide_field(int64_t ncell, const double* f, double* out, double tau, const double* force,
+                          int point_feq) {
+    init_constants();
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < ncell; ++c) {
+        double fc[Q], oc[Q];
+        gather_cell(f, ncell, c, fc);
+        double fx = 0, fy = 0, fz = 0;
+        if (force) {
+            fx = force[c];
+            fy = force[ncell + c];
+            fz = force[2 * ncell + c];
+        }
+        collide_one_form(fc, oc, tau, force != NULL, fx, fy, fz, point_feq);
+        for (int i = 0; i < Q; ++i) out[i * ncell + c] = oc[i];
+    }
+}
+
+/* The moment-encoded variant's collision (ME/shaders/lbm_collide.comp:36-76):
+ * bgk.py's collision with the equilibrium in the division form.  The shader's
+ * force is (force_x, 0, 0), entering u_eq as mx / rho + 0.5 force_x / rho and
+ * the Guo term as prefactor w_i (term_x force_x); each component of `force`
+ * enters as the x component does there, which for (fx, 0, 0) gives the
+ * shader's values. */
+void orc_me_collide(int64_t ncell, const double* f, double* out, double tau, const double* force) {
+    collide_field(ncell, f, out, tau, force, 1);
 }
 
 /* The collision half of bgk.py:45-110 over a whole field (no streaming). */
@@ -784,19 +828,25 @@ void orc_me_decode(int64_t ncell, const uint16_t* codes, const double* lo, const
     }
 }
 
-/* The quantized trajectory: encode f, then `steps` times decode, bgk_step,
- * (y-walls), encode; f is returned decoded.  work: 2 * 19 * ncell doubles. */
-void orc_run_me(int32_t nx, int32_t ny, int32_t nz, double* f, double tau, const double* force,
-                int32_t ywalls, const double* uw_top, const double* uw_bot, int32_t steps, const double* lo,
-                const double* hi, uint16_t* codes, double* work) {
+/* The quantized trajectory continued from `codes` (ME/src/lbm_me.cpp:386-409):
+ * `steps` times decode (lbm_decode.comp), collide (lbm_collide.comp: the
+ * division-form equilibrium), stream and, with ywalls, bounce back
+ * (lbm_stream_bounce.comp: a pull, then the half-way y-walls; for resting
+ * walls this is bgk.py's stream + apply_bounce_back_y_walls to the bit, and
+ * moving walls take apply_bounce_back_y_walls' momentum term), encode
+ * (lbm_encode.comp).  The final codes stay in `codes` and f receives them
+ * decoded.  work: 2 * 19 * ncell doubles. */
+void orc_run_me_codes(int32_t nx, int32_t ny, int32_t nz, uint16_t* codes, double tau, const double* force,
+                      int32_t ywalls, const double* uw_top, const double* uw_bot, int32_t steps, const double* lo,
+                      const double* hi, double* f, double* work) {
     const int64_t ncell = (int64_t)nx * ny * nz;
     const int64_t n = (int64_t)Q * ncell;
     double* a = work;
     double* b = work + n;
-    orc_me_encode(ncell, f, lo, hi, codes);
     for (int32_t s = 0; s < steps; ++s) {
         orc_me_decode(ncell, codes, lo, hi, f);
-        orc_bgk_step(nx, ny, nz, f, a, tau, force, b);
+        orc_me_collide(ncell, f, b, tau, force);
+        orc_stream(nx, ny, nz, b, a);
         if (ywalls)
             orc_bounce_back_y_walls(nx, ny, nz, a, f, uw_top, uw_bot);
         else
@@ -804,6 +854,15 @@ void orc_run_me(int32_t nx, int32_t ny, int32_t nz, double* f, double tau, const
         orc_me_encode(ncell, f, lo, hi, codes);
     }
     orc_me_decode(ncell, codes, lo, hi, f);
+}
+
+/* The quantized trajectory from populations: encode f (ME/src/lbm_me.cpp:376-384),
+ * then orc_run_me_codes. */
+void orc_run_me(int32_t nx, int32_t ny, int32_t nz, double* f, double tau, const double* force,
+                int32_t ywalls, const double* uw_top, const double* uw_bot, int32_t steps, const double* lo,
+                const double* hi, uint16_t* codes, double* work) {
+    orc_me_encode((int64_t)nx * ny * nz, f, lo, hi, codes);
+    orc_run_me_codes(nx, ny, nz, codes, tau, force, ywalls, uw_top, uw_bot, steps, lo, hi, f, work);
 }
 
 /* Threads of the parallel loops (OpenMP): the current maximum, and a setter

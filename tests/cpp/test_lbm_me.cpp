@@ -163,12 +163,35 @@ void t_quantized_tracks_f64() {
     for (size_t k = 0; k < q.frames.back().u.size(); ++k)
         eu = std::max(eu, std::fabs(q.frames.back().u[k] - f.frames.back().u[k]));
     CHECK(e <= 1e-2);
-    // The codes do quantize.  The witness is u: this channel's density stays 1
-    // to within its 1e-12 range floor, below what the FP32 decode resolves, so
-    // the decoded rho equals the FP64 rho exactly.
+    // The codes do quantize; the witness is u (this channel's density stays 1
+    // to within its 1e-12 range floor).
     CHECK(eu > 0.0);
     CHECK(eu <= 1e-2);
     CHECK(q.determinism_witness_sha256 != f.determinism_witness_sha256);
+}
+
+// Both arithmetics of a quantized run stay within the lbm-quantized
+// tolerance (err <= 1e-5 + 5e-2 |parent|) of the FP64 run on every frame.
+void t_quantized_arithmetics() {
+    LbmConfig c = small(true, 0.7, 1e-5);
+    c.capture_interval = 6;
+    const LbmResult r = run_lbm(c, nullptr, QuantArithmetic::reference_fp64);
+    const LbmResult e = run_lbm(c, nullptr, QuantArithmetic::engine_fp32);
+    const LbmResult d = run_lbm(c, nullptr);  // the two-argument form is reference_fp64
+    c.quantize = false;
+    const LbmResult f = run_lbm(c, nullptr);
+    CHECK(d.determinism_witness_sha256 == r.determinism_witness_sha256);
+    CHECK(e.determinism_witness_sha256 != r.determinism_witness_sha256);
+    CHECK(r.frames.size() == f.frames.size() && e.frames.size() == f.frames.size());
+    bool ok_r = true, ok_e = true;
+    for (size_t n = 0; n < f.frames.size() && n < r.frames.size() && n < e.frames.size(); ++n)
+        for (size_t k = 0; k < f.frames[n].u.size(); ++k) {
+            const double p = f.frames[n].u[k];
+            ok_r = ok_r && std::fabs(r.frames[n].u[k] - p) <= 1e-5 + 5e-2 * std::fabs(p);
+            ok_e = ok_e && std::fabs(e.frames[n].u[k] - p) <= 1e-5 + 5e-2 * std::fabs(p);
+        }
+    CHECK(ok_r);
+    CHECK(ok_e);
 }
 
 void t_frames_and_witness() {
@@ -299,6 +322,7 @@ int main(int argc, char** argv) {
         {"quantized mass within the quantization budget", t_quantized_mass_budget},
         {"(tau, force) sweep: mass conserved, rho finite and positive", t_sweep},
         {"quantized run tracks the FP64 run", t_quantized_tracks_f64},
+        {"reference-FP64 and engine-FP32 quantized arithmetic", t_quantized_arithmetics},
         {"frame schedule and witness", t_frames_and_witness},
         {"configuration errors", t_errors},
     };

@@ -77,7 +77,7 @@ def rest(shape):
 def python_run(amd, storage, lo=None, hi=None):
     """The C++ run's C-ABI calls made from Python: frames (rho, u) and the final populations."""
     lat = amd.Lattice(SHAPE, TAU, storage=storage, boundary="ywalls", force=(FX, 0.0, 0.0))
-    if storage == "me16":
+    if storage in ("me16", "me16ref"):
         lat.set_moment_ranges(lo, hi)
     lat.set_populations(rest(SHAPE))
     frames, s = {}, 0
@@ -115,7 +115,8 @@ def test_capture_equals_python_lattice(dump, mode):
         plo, phi = calibrate_moment_ranges(SHAPE, TAU, STEPS, populations=rest(SHAPE), boundary="ywalls",
                                            force=(FX, 0.0, 0.0))
         assert np.array_equal(plo, lo) and np.array_equal(phi, hi)
-        frames, f = python_run(amd, "me16", lo, hi)
+        # run_lbm steps the codes in the reference variant's FP64 arithmetic
+        frames, f = python_run(amd, "me16ref", lo, hi)
     else:
         frames, f = python_run(amd, "f64")
     for n in FRAME_STEPS:
@@ -147,6 +148,26 @@ def test_f64_capture_is_the_reference_loop(dump):
         st = cap.step(n)
         assert field_scaled_err(st.state["rho"], oracle.density_field(f)) < 1e-13, n
         assert field_scaled_err(st.state["u"], oracle.macroscopic_velocity(f, F)) < 1e-10, n
+
+
+def test_me16_capture_is_the_oracle_quantized_trajectory(dump):
+    """run_lbm(quantize=true) steps the codes in the reference variant's FP64
+    arithmetic: every captured frame's rho and u are those of the oracle's
+    quantized trajectory (decode, bgk_step, y-walls, encode) to the bit, and
+    the witness is the SHA-256 of the oracle's final decoded populations."""
+    from lattice_boltzmann_d3q19_amd.capture import load_capture
+
+    print_case("cpp-me-me16-vs-oracle")
+    runs, lo, hi = results(dump)
+    cap = load_capture(dump / "me16_capture.json")
+    F = np.zeros((3,) + SHAPE)
+    F[0] = FX
+    for n in FRAME_STEPS:
+        q = oracle.run_me(rest(SHAPE), TAU, n, lo, hi, force_lattice=F, ywalls=True)
+        st = cap.step(n)
+        assert np.array_equal(st.state["rho"], oracle.density_field(q)), n
+        assert np.array_equal(st.state["u"], oracle.macroscopic_velocity(q, F)), n
+    assert runs[0]["witness"] == hashlib.sha256(np.ascontiguousarray(q, dtype="<f8").tobytes()).hexdigest()
 
 
 def test_capture_tool(tmp_path):

@@ -13,7 +13,12 @@
 * the canonical moment-encoded capture (1001 frames) against the parent
   frame by frame, its measured max relative error over |u| > 1e-6 near the
   published 1.98e-2, and the Poiseuille structure checks of
-  ME/tests/python/test_frontier_equivalence.py:95-121.
+  ME/tests/python/test_frontier_equivalence.py:95-121;
+* "me16ref" (the variant's own FP64 arithmetic: decode, division-form
+  collision, stream + y-walls, encode): codes, decoded populations and
+  moments equal the oracle's quantized trajectory bit for bit (y-wall channel
+  with a uniform force, periodic lattice with a force field); set_equilibrium,
+  checkpoint / restore, and the refused operations.
 """
 
 from __future__ import annotations
@@ -112,11 +117,13 @@ def test_trajectory_tracks_parent_and_oracle_codec(amd):
     lo, hi = oracle_ranges(frames)
     lat = amd.Lattice(CANON, TAU, storage="me16", boundary="ywalls", force=(FX, 0.0, 0.0))
     lat.set_moment_ranges(lo, hi).set_equilibrium()
-    q = oracle.me_decode(oracle.me_encode(frames[0], lo, hi), lo, hi)
+    codes = oracle.me_encode(frames[0], lo, hi)
+    q = oracle.me_decode(codes, lo, hi)
     for n in range(steps + 1):
         if n:
             lat.step(1)
-            q = oracle.run_me(q, TAU, 1, lo, hi, force_lattice=F, ywalls=True)
+            q, codes = oracle.run_me(None, TAU, 1, lo, hi, force_lattice=F, ywalls=True, codes=codes,
+                                     return_codes=True)
         if n % 20:
             continue
         _, u = lat.moments()
@@ -172,3 +179,88 @@ def test_me16_errors(amd):
         lat.set_moment_ranges(lo - 1, hi)
     with pytest.raises(D3Q19Error, match="EINVAL"):
         amd.Lattice((4, 4, 4), TAU, storage="f32").set_moment_ranges(lo, hi)
+
+
+def _me16ref_case(case):
+    """(shape, f0, F (3, ...) or None, uniform force, ywalls, steps, ranges)."""
+    if case == "poiseuille-ywalls":
+        frames, F = parent_frames(200)
+        return CANON, frames[0], F, (FX, 0.0, 0.0), True, 200, oracle_ranges(frames)
+    shape = (12, 10, 8)
+    rng = np.random.default_rng(7)
+    f0 = oracle.feq_field(1.0 + 0.01 * rng.standard_normal(shape), 0.02 * rng.standard_normal((3,) + shape))
+    F = 1e-4 * rng.standard_normal((3,) + shape)
+    frames = [f0]
+    for _ in range(60):
+        frames.append(oracle.run_reference(frames[-1], TAU, 1, force_lattice=F))
+    return shape, f0, F, None, False, 60, oracle_ranges(frames)
+
+
+@pytest.mark.parametrize("case", ["poiseuille-ywalls", "periodic-force-field"])
+def test_me16ref_equals_oracle_quantized_trajectory_bit_for_bit(amd, case):
+    """'me16ref' steps the codes in the moment-encoded variant's FP64
+    arithmetic (ME/src/lbm_me.cpp:386-409): on every checked frame the stored
+    codes equal the oracle's quantized trajectory's codes, and the decoded
+    populations and moments equal the oracle's to the bit.
+
+    The oracle continues from its own codes between checked frames: decode
+    then encode is not the identity for a moment whose range is narrower than
+    FP64 resolves around its value (the channel's density, range about 1e-12
+    wide around 1), so a trajectory restarted from decoded populations is a
+    different trajectory."""
+    print_case("me16ref-bit-exact", case=case)
+    shape, f0, F, uniform, ywalls, steps, (lo, hi) = _me16ref_case(case)
+    kw = dict(boundary="ywalls" if ywalls else "periodic")
+    kw["force"] = uniform if uniform is not None else F
+    lat = amd.Lattice(shape, TAU, storage="me16ref", **kw).set_moment_ranges(lo, hi).set_populations(f0)
+    done = 0
+    codes = oracle.me_encode(f0, lo, hi)
+    for n in sorted(set(range(0, steps + 1, 20)) | {steps}):
+        lat.step(n - done)
+        q, codes = oracle.run_me(None, TAU, n - done, lo, hi, force_lattice=F, ywalls=ywalls, codes=codes,
+                                 return_codes=True)
+        done = n
+        raw, local, count = lat.raw_state()
+        assert local and count == n
+        assert np.array_equal(raw.view(np.uint16).reshape(codes.shape), codes), n
+        got = lat.populations()
+        assert np.array_equal(got, q), (n, np.abs(got - q).max())
+        rho, u = lat.moments()
+        assert np.array_equal(rho, oracle.density_field(q)), n
+        assert np.array_equal(u, oracle.macroscopic_velocity(q, F)), n
+    lat.close()
+
+
+def test_me16ref_equilibrium_checkpoint_and_refusals(amd, tmp_path):
+    from lattice_boltzmann_d3q19_amd import D3Q19Error
+
+    print_case("me16ref-equilibrium-refusals")
+    shape = (8, 6, 4)
+    frames, _ = parent_frames(20, shape)
+    lo, hi = oracle_ranges(frames)
+    rng = np.random.default_rng(3)
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    u = 0.02 * rng.standard_normal((3,) + shape)
+    lat = amd.Lattice(shape, TAU, storage="me16ref")
+    with pytest.raises(D3Q19Error, match="ESTATE"):
+        lat.set_equilibrium(rho, u)
+    lat.set_moment_ranges(lo, hi).set_equilibrium(rho, u)
+    want = oracle.me_decode(oracle.me_encode(oracle.feq_field(rho, u), lo, hi), lo, hi)
+    assert np.array_equal(lat.populations(), want)
+    # a checkpoint of the codes continues bit for bit
+    lat.step(5)
+    ck = lat.checkpoint(tmp_path / "me16ref.npz")
+    lat.step(5)
+    other = amd.Lattice(shape, TAU, storage="me16ref")
+    other.restore(ck)
+    other.step(5)
+    assert np.array_equal(other.populations(), lat.populations())
+    other.close()
+    for call in (lat.diagnostics, lat.moment_range, lat.enstrophy):
+        with pytest.raises(D3Q19Error, match="EINVAL"):
+            call()
+    with pytest.raises(D3Q19Error, match="EINVAL"):
+        lat.set_brinkman(np.zeros(shape))
+    lat.close()
+    with pytest.raises(D3Q19Error, match="EINVAL"):
+        amd.Lattice(shape, TAU, storage="me16ref", boundary="halfway")

@@ -100,11 +100,13 @@ def test_canonical_poiseuille_reproduces_published_measurement():
         for _ in range(steps):
             parent.append(oracle.run_reference(parent[-1], tau, 1, force_lattice=F, ywalls=True))
         lo, hi = ranges_over(parent)
-        q = oracle.me_decode(oracle.me_encode(f, lo, hi), lo, hi)
+        codes = oracle.me_encode(f, lo, hi)
+        q = oracle.me_decode(codes, lo, hi)
         rho_abs = u_abs = max_rel = 0.0
         for n in range(steps + 1):
-            if n:
-                q = oracle.run_me(q, tau, 1, lo, hi, force_lattice=F, ywalls=True)
+            if n:  # one unbroken trajectory of codes, as the variant runs it
+                q, codes = oracle.run_me(None, tau, 1, lo, hi, force_lattice=F, ywalls=True, codes=codes,
+                                         return_codes=True)
             rp, up = oracle.density_field(parent[n]), oracle.macroscopic_velocity(parent[n], F)
             rq, uq = oracle.density_field(q), oracle.macroscopic_velocity(q, F)
             e = np.abs(uq - up)
@@ -120,3 +122,74 @@ def test_canonical_poiseuille_reproduces_published_measurement():
     assert max_rel == pytest.approx(1.98e-2, abs=5e-4)
     assert 1e-6 < u_abs < 3.2e-6
     assert rho_abs < 5e-14
+
+
+def _perturbed(shape, seed):
+    rng = np.random.default_rng(seed)
+    return oracle.feq_field(1 + 0.02 * rng.standard_normal(shape), 0.05 * rng.standard_normal((3,) + shape))
+
+
+def _point_form_collision(f, tau):
+    """Unforced BGK relaxation toward the point function's equilibrium
+    (oracle.feq, pinned against the reference's golden vectors) at the
+    density and velocity that density_field / macroscopic_velocity give."""
+    rho = oracle.density_field(f).ravel()
+    u = oracle.macroscopic_velocity(f).reshape(3, -1)
+    feq = np.array([oracle.feq(rho[c], u[:, c]) for c in range(rho.size)]).T.reshape(f.shape)
+    return f - (f - feq) / tau
+
+
+def test_me_collide_relaxes_toward_the_point_form_equilibrium():
+    """The variant's collision (shaders/lbm_collide.comp) uses the division-form
+    equilibrium: unforced, it is the relaxation toward the point function's
+    f_eq to the bit, and it differs from bgk.py's field-form collision in the
+    last bit on a few percent of the entries (never by more than 4 ulp)."""
+    f = _perturbed((6, 5, 4), 11)
+    tau = 0.8
+    got = oracle.me_collide(f, tau)
+    assert np.array_equal(got, _point_form_collision(f, tau))
+    field = oracle.collide(f, tau)
+    diff = got != field
+    assert 0.01 < diff.mean() < 0.5
+    assert np.all(np.abs(got - field) <= 4 * np.spacing(np.abs(field)))
+    # with a force, the Guo shift and term are bgk.py's: same mass, same momentum gain
+    F = 1e-4 * np.random.default_rng(5).standard_normal((3,) + f.shape[1:])
+    gf = oracle.me_collide(f, tau, F)
+    assert np.abs(oracle.density_field(gf) - oracle.density_field(f)).max() < 1e-15
+    gain = oracle.momentum_field(gf) - oracle.momentum_field(f)
+    assert np.abs(gain - F).max() < 1e-15
+
+
+def test_planted_field_form_in_the_variant_collision_is_caught():
+    f = _perturbed((6, 5, 4), 11)
+    with oracle.planted("ME_FIELD_FEQ"):
+        got = oracle.me_collide(f, 0.8)
+    assert not np.array_equal(got, _point_form_collision(f, 0.8))
+
+
+def test_run_me_continues_from_codes():
+    """A trajectory continued from its codes is the uninterrupted one; restarted
+    from its decoded populations it is another, once a moment's range is
+    narrower than FP64 resolves around its value (the resting channel's
+    density: a 1e-12-wide range around 1)."""
+    shape, tau = (16, 8, 3), 0.7
+    F = np.zeros((3,) + shape)
+    F[0] = 1e-5
+    f0 = oracle.feq_field(np.ones(shape), np.zeros((3,) + shape))
+    frames = [f0]
+    for _ in range(40):
+        frames.append(oracle.run_reference(frames[-1], tau, 1, force_lattice=F, ywalls=True))
+    lo, hi = ranges_over(frames)
+    assert hi[0] - lo[0] < 1e-11
+    kw = dict(force_lattice=F, ywalls=True)
+    q40, c40 = oracle.run_me(f0, tau, 40, lo, hi, return_codes=True, **kw)
+    q20, c20 = oracle.run_me(f0, tau, 20, lo, hi, return_codes=True, **kw)
+    q, c = oracle.run_me(None, tau, 20, lo, hi, codes=c20, return_codes=True, **kw)
+    assert np.array_equal(c, c40) and np.array_equal(q, q40)
+    assert np.array_equal(oracle.run_me(None, tau, 0, lo, hi, codes=c20), q20)
+    _, cr = oracle.run_me(q20, tau, 20, lo, hi, return_codes=True, **kw)
+    assert not np.array_equal(cr, c40)
+    # strided range arrays (views) are read as their values
+    rng2 = np.stack([lo, hi], axis=1)
+    assert np.array_equal(oracle.run_me(f0, tau, 5, rng2[:, 0], rng2[:, 1], **kw),
+                          oracle.run_me(f0, tau, 5, lo.copy(), hi.copy(), **kw))
