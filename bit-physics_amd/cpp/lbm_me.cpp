@@ -79,8 +79,12 @@ struct Channel {
         }
     }
     void populations(std::vector<double>& f) { check(d3q19_read_populations(h, f.data(), D3Q19_F64, 0), "read_populations"); }
-    void moment_range(std::array<double, kQ>& lo, std::array<double, kQ>& hi) {
-        check(d3q19_moment_range(h, lo.data(), hi.data()), "moment_range");
+    // stored: the moments of the stored state (what D3Q19_ME16 encodes), else of f^n
+    void moment_range(std::array<double, kQ>& lo, std::array<double, kQ>& hi, bool stored = false) {
+        if (stored)
+            check(d3q19_moment_range_stored(h, lo.data(), hi.data()), "moment_range_stored");
+        else
+            check(d3q19_moment_range(h, lo.data(), hi.data()), "moment_range");
     }
 };
 
@@ -220,21 +224,21 @@ void reference_step(std::vector<double>& f, const LbmConfig& cfg) {
     lat.populations(f);
 }
 
-QuantRanges calibrate_ranges(const LbmConfig& cfg, const MomentBasis& basis) {
-    validate(cfg);
-    if (basis.m != build_moment_basis().m)
-        throw std::invalid_argument("calibrate_ranges: the engine's codec needs build_moment_basis()'s basis");
+namespace {
+// The variant's range rule over the FP64 run's f^n, or (stored) over its
+// stored post-collision states, the states a D3Q19_ME16 lattice encodes.
+QuantRanges calibrate(const LbmConfig& cfg, bool stored) {
     Channel lat(cfg, D3Q19_F64);
     lat.load(initial_rest_state(cfg));
     QuantRanges r;
-    lat.moment_range(r.lo, r.hi);
+    lat.moment_range(r.lo, r.hi, stored);
     std::array<double, kQ> lo, hi;
     // The ranges must cover the whole run, not only the warm-up: a start-up
     // flow keeps accelerating, and a clamped momentum moment biases u.
     const uint32_t horizon = std::max(cfg.warmup_steps, cfg.steps);
     for (uint32_t s = 0; s < horizon; ++s) {
         lat.step(1);
-        lat.moment_range(lo, hi);
+        lat.moment_range(lo, hi, stored);
         for (int k = 0; k < kQ; ++k) {
             r.lo[k] = std::min(r.lo[k], lo[k]);
             r.hi[k] = std::max(r.hi[k], hi[k]);
@@ -246,6 +250,14 @@ QuantRanges calibrate_ranges(const LbmConfig& cfg, const MomentBasis& basis) {
         r.hi[k] += cfg.range_margin * span;
     }
     return r;
+}
+}  // namespace
+
+QuantRanges calibrate_ranges(const LbmConfig& cfg, const MomentBasis& basis) {
+    validate(cfg);
+    if (basis.m != build_moment_basis().m)
+        throw std::invalid_argument("calibrate_ranges: the engine's codec needs build_moment_basis()'s basis");
+    return calibrate(cfg, false);
 }
 
 LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest) {
@@ -260,7 +272,8 @@ LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_man
     const size_t n = ncell(cfg);
     const MomentBasis basis = build_moment_basis();
     QuantRanges ranges{};
-    if (cfg.quantize) ranges = calibrate_ranges(cfg, basis);
+    if (cfg.quantize)
+        ranges = arithmetic == QuantArithmetic::reference_fp64 ? calibrate_ranges(cfg, basis) : calibrate(cfg, true);
     const std::vector<double> f0 = initial_rest_state(cfg);
 
     // One trajectory; frames (0, every capture_interval, the last step) when

@@ -84,10 +84,13 @@ struct d3q19_lattice {
     void* uscratch = nullptr;  // read-back velocity (3, ncell) in the compute type (enstrophy)
     bool has_state = false;
     bool local = false;  // buffer cur holds pre-collision f^n in place
-    // D3Q19_ME16: buf[0] holds the moment codes, buf[1] the FP32 work lattice.
-    // D3Q19_ME16_REF: buf[0] the codes (moment-major, k * ncell + cell), buf[1]
-    // and ref_b two FP64 population arrays in the reference layout.
+    // D3Q19_ME16: buf[0] and buf[1] hold moment codes in the population layout
+    // (2-byte elements, moment k in place of direction k); mq their
+    // quantization constants (device).  D3Q19_ME16_REF: buf[0] the codes
+    // (moment-major, k * ncell + cell), buf[1] and ref_b two FP64 population
+    // arrays in the reference layout.
     bool me = false;
+    MeQuant* mq = nullptr;
     bool me_ref = false;
     void* ref_b = nullptr;
     size_t codes_bytes = 0;
@@ -127,6 +130,7 @@ Params<C> make_params(const d3q19_lattice* L) {
     p.solid = L->cfg.boundary == D3Q19_BC_HALFWAY ? L->solid : nullptr;
     p.local = L->local ? 1 : 0;
     set_zwall_terms(p);
+    p.me = L->mq;
     p.seed = (uint32_t)L->steps * 0x9e3779b9u + 0x7f4a7c15u;
     return p;
 }
@@ -216,133 +220,42 @@ int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     return D3Q19_OK;
 }
 
-// ---- moment-encoded storage (D3Q19_ME16) -----------------------------------
-
-MeCodes make_codes(const d3q19_lattice* L) {
-    MeCodes mc;
-    std::memset(&mc, 0, sizeof(mc));
-    mc.dir_bytes = (uint32_t)(L->dir_bytes / 2);
-    mc.row_bytes = L->row_bytes / 2;
-    for (int k = 0; k < Q; ++k) {
-        double mw = 0.0;  // moment k of the rest state, (M w)_k
-        for (int i = 0; i < Q; ++i) mw += me_mono(k, i) * wgt<double>(i);
-        const double span = L->me_hi[k] - L->me_lo[k];
-        mc.lo[k] = (float)(L->me_lo[k] - mw);
-        mc.step[k] = (float)(span / 65535.0);
-        mc.scale[k] = (float)(65535.0 / span);
-    }
-    return mc;
-}
-
-float* me_work(const d3q19_lattice* L) { return (float*)L->buf[1]; }
-uint16_t* me_codes(const d3q19_lattice* L) { return (uint16_t*)L->buf[0]; }
-
-template <bool ROWS>
-int me_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
-    const StepGrid sg = step_grid<TF32>(L);
-    const MeCodes mc = make_codes(L);
-    for (int32_t s = 0; s < n; ++s) {
-        const Params<float> p = make_params<float>(L);
-        const bool forced = p.force_mode != D3Q19_FORCE_NONE || p.brink != nullptr;
-        float* r = (s == n - 1) ? rho : nullptr;
-        float* v = (s == n - 1) ? u : nullptr;
-        if (forced)
-            k_me_collide<true, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc, r, v);
-        else
-            k_me_collide<false, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc, r, v);
-        HIPCHK(hipGetLastError());
-        switch (L->cfg.boundary) {
-            case D3Q19_BC_PERIODIC:
-                k_me_stream_encode<BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L),
-                                                                                          p, mc);
-                break;
-            case D3Q19_BC_YWALLS:
-                k_me_stream_encode<BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p,
-                                                                                        mc);
-                break;
-            default:
-                k_me_stream_encode<BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L),
-                                                                                         p, mc);
-                break;
-        }
-        HIPCHK(hipGetLastError());
-        L->steps++;
-    }
-    return D3Q19_OK;
-}
-
-// Decode the codes into the work lattice as an in-place FP32 state.
-int me_decode(d3q19_lattice* L) {
-    const StepGrid sg = step_grid<TF32>(L);
-    const Params<float> p = make_params<float>(L);
-    const MeCodes mc = make_codes(L);
-    if (L->rows)
-        k_me_decode_local<true><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc);
-    else
-        k_me_decode_local<false><<<sg.grid, sg.block, 0, L->stream>>>(me_codes(L), me_work(L), p, mc);
-    HIPCHK(hipGetLastError());
-    return D3Q19_OK;
-}
-
 template <class Tag>
 int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
-    if constexpr (std::is_same<Tag, TME16>::value)
-        return L->rows ? me_step_l<true>(L, n, rho, u) : me_step_l<false>(L, n, rho, u);
-    else
-        return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
+    return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
 }
 
 template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        if (!L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
-        const Params<float> p = make_params<float>(L);
-        const StepGrid sg = step_grid<TF32>(L);
-        const MeCodes mc = make_codes(L);
-        if (L->rows) {
-            k_load<TF32, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, me_work(L), p);
-            k_me_encode_local<true><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p, mc);
-        } else {
-            k_load<TF32, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, me_work(L), p);
-            k_me_encode_local<false><<<sg.grid, sg.block, 0, L->stream>>>(me_work(L), me_codes(L), p, mc);
-        }
-        HIPCHK(hipGetLastError());
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            k_load<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
-        else
-            k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
-        HIPCHK(hipGetLastError());
-    }
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        k_load<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+    else
+        k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
+    HIPCHK(hipGetLastError());
     L->has_state = true;
     L->local = true;
     L->steps = 0;
     return D3Q19_OK;
 }
 
-// feq(rho, u) straight into the lattice storage (all storages but ME16, whose
-// codes are made from populations).
+// feq(rho, u) straight into the lattice storage.
 template <class Tag, class In>
 int launch_init_eq(d3q19_lattice* L, const In* rho_dev, const In* u_dev) {
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        return fail(D3Q19_EINVAL, "D3Q19_ME16 equilibrium is loaded through populations");
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            k_load_eq<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
-        else
-            k_load_eq<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
-        HIPCHK(hipGetLastError());
-        L->has_state = true;
-        L->local = true;
-        L->steps = 0;
-        return D3Q19_OK;
-    }
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        k_load_eq<Tag, In, true><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
+    else
+        k_load_eq<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
+    HIPCHK(hipGetLastError());
+    L->has_state = true;
+    L->local = true;
+    L->steps = 0;
+    return D3Q19_OK;
 }
 
 template <class Tag, class Out, bool ROWS>
@@ -363,25 +276,13 @@ void launch_readback_l(d3q19_lattice* L, const StepGrid& sg, const Params<typena
 
 template <class Tag, class Out>
 int launch_readback(d3q19_lattice* L, Out* f, Out* rho, Out* u) {
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        // decode, then read the work lattice back as an in-place FP32 state
-        if (int e = me_decode(L)) return e;
-        Params<float> p = make_params<float>(L);
-        p.local = 1;
-        const StepGrid sg = step_grid<TF32>(L);
-        if (L->rows)
-            launch_readback_l<TF32, Out, true>(L, sg, p, f, rho, u, me_work(L));
-        else
-            launch_readback_l<TF32, Out, false>(L, sg, p, f, rho, u, me_work(L));
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            launch_readback_l<Tag, Out, true>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
-        else
-            launch_readback_l<Tag, Out, false>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
-    }
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        launch_readback_l<Tag, Out, true>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
+    else
+        launch_readback_l<Tag, Out, false>(L, sg, p, f, rho, u, buf_of<Tag>(L, L->cur));
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -404,29 +305,15 @@ void launch_diag_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename C
 
 template <class Tag>
 int launch_diag(d3q19_lattice* L) {
-    uint32_t nblk;
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        if (int e = me_decode(L)) return e;
-        Params<float> p = make_params<float>(L);
-        p.local = 1;
-        const StepGrid sg = step_grid<TF32>(L);
-        if (L->rows)
-            launch_diag_l<TF32, true>(L, sg, p, me_work(L));
-        else
-            launch_diag_l<TF32, false>(L, sg, p, me_work(L));
-        nblk = sg.nblocks;
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            launch_diag_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
-        else
-            launch_diag_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
-        nblk = sg.nblocks;
-    }
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        launch_diag_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+    else
+        launch_diag_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
     HIPCHK(hipGetLastError());
-    k_sum_partials<<<NDIAG, BLOCK, 0, L->stream>>>(L->partials, nblk, NDIAG, L->dout);
+    k_sum_partials<<<NDIAG, BLOCK, 0, L->stream>>>(L->partials, sg.nblocks, NDIAG, L->dout);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -560,31 +447,21 @@ void launch_moment_range_l(d3q19_lattice* L, const StepGrid& sg, const Params<ty
     }
 }
 
+// stored: the moments of the stored state as it lies in the buffer (the
+// post-collision populations of the last step, or a loaded state) instead of
+// the state f^n the read-back gives.
 template <class Tag>
-int launch_moment_range(d3q19_lattice* L) {
-    uint32_t nblk;
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        if (int e = me_decode(L)) return e;
-        Params<float> p = make_params<float>(L);
-        p.local = 1;
-        const StepGrid sg = step_grid<TF32>(L);
-        if (L->rows)
-            launch_moment_range_l<TF32, true>(L, sg, p, me_work(L));
-        else
-            launch_moment_range_l<TF32, false>(L, sg, p, me_work(L));
-        nblk = sg.nblocks;
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            launch_moment_range_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
-        else
-            launch_moment_range_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
-        nblk = sg.nblocks;
-    }
+int launch_moment_range(d3q19_lattice* L, bool stored) {
+    typedef typename Codec<Tag>::C C;
+    Params<C> p = make_params<C>(L);
+    if (stored) p.local = 1;
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        launch_moment_range_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+    else
+        launch_moment_range_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
     HIPCHK(hipGetLastError());
-    k_moment_range_final<<<2 * Q, 256, 0, L->stream>>>(L->mr_partials, nblk, L->mr_out);
+    k_moment_range_final<<<2 * Q, 256, 0, L->stream>>>(L->mr_partials, sg.nblocks, L->mr_out);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -609,29 +486,15 @@ void launch_health_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename
 
 template <class Tag>
 int launch_health(d3q19_lattice* L) {
-    uint32_t nblk;
-    if constexpr (std::is_same<Tag, TME16>::value) {
-        if (int e = me_decode(L)) return e;
-        Params<float> p = make_params<float>(L);
-        p.local = 1;
-        const StepGrid sg = step_grid<TF32>(L);
-        if (L->rows)
-            launch_health_l<TF32, true>(L, sg, p, me_work(L));
-        else
-            launch_health_l<TF32, false>(L, sg, p, me_work(L));
-        nblk = sg.nblocks;
-    } else {
-        typedef typename Codec<Tag>::C C;
-        const Params<C> p = make_params<C>(L);
-        const StepGrid sg = step_grid<Tag>(L);
-        if (L->rows)
-            launch_health_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
-        else
-            launch_health_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
-        nblk = sg.nblocks;
-    }
+    typedef typename Codec<Tag>::C C;
+    const Params<C> p = make_params<C>(L);
+    const StepGrid sg = step_grid<Tag>(L);
+    if (L->rows)
+        launch_health_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
+    else
+        launch_health_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
     HIPCHK(hipGetLastError());
-    k_health_final<<<NHEALTH, 256, 0, L->stream>>>(L->partials, nblk, L->dout);
+    k_health_final<<<NHEALTH, 256, 0, L->stream>>>(L->partials, sg.nblocks, L->dout);
     HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
@@ -649,6 +512,29 @@ int launch_health(d3q19_lattice* L) {
 extern "C" int d3q19_destroy(d3q19_lattice* L);
 
 namespace {
+
+// Quantization constants of a D3Q19_ME16 lattice from its ranges: the
+// deviation-moment grid lo'_k = lo_k - (M w)_k, step_k = (hi_k - lo_k) / 65535
+// and the decode offsets L_i = sum_k Minv[i][k] lo'_k, formed in FP64.
+MeQuant me_quant(const double lo[19], const double hi[19]) {
+    MeQuant q;
+    double lod[Q];
+    for (int k = 0; k < Q; ++k) {
+        double mw = 0.0;  // moment k of the rest state
+        for (int i = 0; i < Q; ++i) mw += me_mono(k, i) * wgt<double>(i);
+        const double span = hi[k] - lo[k];
+        lod[k] = lo[k] - mw;
+        q.step[k] = (float)(span / 65535.0);
+        q.lo[k] = (float)lod[k];
+        q.sc[k] = (float)(65535.0 / span);
+    }
+    for (int i = 0; i < Q; ++i) {
+        double l = 0.0;
+        for (int k = 0; k < Q; ++k) l += 0.25 * ME_MINV4[i][k] * lod[k];
+        q.L[i] = (float)l;
+    }
+    return q;
+}
 
 MeRef make_me_ref(const d3q19_lattice* L) {
     MeRef r;
@@ -668,7 +554,6 @@ int create_me_ref(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     d3q19_lattice* L = new d3q19_lattice();
     L->cfg = *cfg;
     L->ncell = (uint32_t)((uint64_t)cfg->nx * cfg->ny * cfg->nz);
-    L->me = true;
     L->me_ref = true;
     L->rows = false;
     L->esize = 2;
@@ -856,9 +741,7 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return fail(D3Q19_EINVAL, "extents must be >= 1");
     const uint64_t ncell = (uint64_t)cfg->nx * cfg->ny * cfg->nz;
     const bool me = cfg->storage == D3Q19_ME16;
-    // the moment-encoded lattice lays out its FP32 work lattice like an F32
-    // lattice and its codes in the same layout at 2 bytes per element
-    const size_t es = me ? 4 : storage_bytes(cfg->storage);
+    const size_t es = storage_bytes(cfg->storage);
     if (es == 0) return fail(D3Q19_EINVAL, "storage must be F64, F32, F16, BF16, BF16SR or ME16");
     if (ncell >= (1ull << 32)) return fail(D3Q19_EINVAL, "lattice too large (nx * ny * nz must be < 2^32)");
     if (cfg->nx > 65535 || cfg->ny > 65535) return fail(D3Q19_EINVAL, "nx and ny must be <= 65535");
@@ -900,20 +783,19 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     // linear 256-thread grids (ncell / 256)
     const uint32_t nb = (uint32_t)std::max<uint64_t>(
         nblocks(ncell), L->rows ? (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64) : 0);
-    const size_t buf0_bytes = me ? lat_bytes / 2 : lat_bytes;  // codes, or lattice 0
-    if (hipMalloc(&L->buf[0], buf0_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
+    if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
         hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
-        hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess) {
+        hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess ||
+        (me && hipMalloc((void**)&L->mq, sizeof(MeQuant)) != hipSuccess)) {
         d3q19_destroy(L);
         return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
     }
-    L->bytes = buf0_bytes + lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64;
-    if (me) L->codes_bytes = buf0_bytes;
+    L->bytes = 2 * lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64 + (me ? sizeof(MeQuant) : 0);
     // Row padding (z >= nz of a padded row) is never written by a kernel:
     // zero it once, so that the raw state (read_state, checkpoints) is a
     // function of the lattice's content alone.
     if (L->rows && L->nzp != (uint32_t)cfg->nz) {
-        if (hipMemset(L->buf[0], 0, buf0_bytes) != hipSuccess || hipMemset(L->buf[1], 0, lat_bytes) != hipSuccess) {
+        if (hipMemset(L->buf[0], 0, lat_bytes) != hipSuccess || hipMemset(L->buf[1], 0, lat_bytes) != hipSuccess) {
             d3q19_destroy(L);
             return fail(D3Q19_EHIP, "clearing the lattice padding failed");
         }
@@ -926,7 +808,7 @@ int d3q19_destroy(d3q19_lattice* L) {
     if (!L) return D3Q19_OK;
     (void)hipStreamSynchronize(L->stream);
     void* ps[] = {L->buf[0], L->buf[1],   L->force,       L->brink,  L->solid, L->partials,
-                  L->dout,   L->uscratch, L->mr_partials, L->mr_out, L->ref_b};
+                  L->dout,   L->uscratch, L->mr_partials, L->mr_out, L->ref_b, L->mq};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete L;
@@ -945,6 +827,7 @@ int d3q19_set_populations(d3q19_lattice* L, const void* f, int32_t dt, int32_t o
     if (int e = check_dtype(dt)) return e;
     if (!f) return fail(D3Q19_EINVAL, "null populations");
     if (L->me_ref) return me_ref_set_populations(L, f, dt, on_device);
+    if (L->me && !L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
     const size_t n = (size_t)Q * L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf tmp;
@@ -970,6 +853,7 @@ int d3q19_set_equilibrium(d3q19_lattice* L, const void* rho, const void* u, int3
     if (int e = check_dtype(dt)) return e;
     if (!u) return fail(D3Q19_EINVAL, "null velocity");
     if (L->me_ref) return me_ref_set_equilibrium(L, rho, u, dt, on_device);
+    if (L->me && !L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
     const size_t nc = L->ncell;
     const size_t es = dt == D3Q19_F64 ? 8 : 4;
     DevBuf drho, du, df;
@@ -1229,7 +1113,7 @@ int d3q19_moment_basis(int32_t m[361], int32_t minv4[361]) {
 
 int d3q19_set_moment_ranges(d3q19_lattice* L, const double lo[19], const double hi[19]) {
     if (int e = check_lat(L)) return e;
-    if (!L->me) return fail(D3Q19_EINVAL, "moment ranges apply to D3Q19_ME16 lattices");
+    if (!L->me && !L->me_ref) return fail(D3Q19_EINVAL, "moment ranges apply to D3Q19_ME16 and D3Q19_ME16_REF lattices");
     if (!lo || !hi) return fail(D3Q19_EINVAL, "null ranges");
     if (L->has_state) return fail(D3Q19_ESTATE, "set the moment ranges before loading a state");
     for (int k = 0; k < Q; ++k)
@@ -1239,12 +1123,18 @@ int d3q19_set_moment_ranges(d3q19_lattice* L, const double lo[19], const double 
         L->me_lo[k] = lo[k];
         L->me_hi[k] = hi[k];
     }
+    if (L->mq) {
+        const MeQuant q = me_quant(lo, hi);
+        HIPCHK(hipMemcpyAsync(L->mq, &q, sizeof(q), hipMemcpyHostToDevice, L->stream));
+        HIPCHK(hipStreamSynchronize(L->stream));
+    }
     L->me_ranges = true;
     return D3Q19_OK;
 }
 
 
-int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
+namespace {
+int moment_range(d3q19_lattice* L, double lo[19], double hi[19], bool stored) {
     if (int e = check_lat(L)) return e;
     if (!lo || !hi) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
@@ -1264,7 +1154,7 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
         }
         L->bytes += pbytes + 2 * Q * sizeof(double);
     }
-    if (int rc = D3Q19_DISPATCH(L, launch_moment_range, L)) return rc;
+    if (int rc = D3Q19_DISPATCH(L, launch_moment_range, L, stored)) return rc;
     double out[2 * Q];
     HIPCHK(hipMemcpyAsync(out, L->mr_out, sizeof(out), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
@@ -1274,17 +1164,24 @@ int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) {
     }
     return D3Q19_OK;
 }
+}  // namespace
+
+int d3q19_moment_range(d3q19_lattice* L, double lo[19], double hi[19]) { return moment_range(L, lo, hi, false); }
+
+int d3q19_moment_range_stored(d3q19_lattice* L, double lo[19], double hi[19]) {
+    return moment_range(L, lo, hi, true);
+}
 
 size_t d3q19_state_bytes(const d3q19_lattice* L) {
     if (!L) return 0;
-    return L->me ? L->codes_bytes : L->lat_bytes;
+    return L->me_ref ? L->codes_bytes : L->lat_bytes;
 }
 
 int d3q19_read_state(d3q19_lattice* L, void* dst, int32_t* local, int64_t* steps) {
     if (int e = check_lat(L)) return e;
     if (!dst) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
-    const void* src = L->me ? L->buf[0] : L->buf[L->cur];
+    const void* src = L->me_ref ? L->buf[0] : L->buf[L->cur];
     HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     if (local) *local = L->local ? 1 : 0;
@@ -1296,12 +1193,13 @@ int d3q19_write_state(d3q19_lattice* L, const void* src, int32_t local, int64_t 
     if (int e = check_lat(L)) return e;
     if (!src) return fail(D3Q19_EINVAL, "null state");
     if (steps < 0) return fail(D3Q19_EINVAL, "negative step count");
-    if (L->me && !L->me_ranges) return fail(D3Q19_ESTATE, "D3Q19_ME16 lattice needs d3q19_set_moment_ranges first");
-    void* dst = L->me ? L->buf[0] : L->buf[L->cur];
+    if ((L->me || L->me_ref) && !L->me_ranges)
+        return fail(D3Q19_ESTATE, "a moment-encoded lattice needs d3q19_set_moment_ranges first");
+    void* dst = L->me_ref ? L->buf[0] : L->buf[L->cur];
     HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyHostToDevice, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
     L->has_state = true;
-    L->local = L->me ? true : local != 0;
+    L->local = L->me_ref ? true : local != 0;
     L->steps = steps;
     return D3Q19_OK;
 }

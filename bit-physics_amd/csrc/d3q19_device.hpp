@@ -59,6 +59,7 @@ struct TF32 {};
 struct TF16 {};   // IEEE half of (f_i - w_i) * (2048 / w_i)
 struct TBF16 {};  // bfloat16 of (f_i - w_i), rounded to nearest even
 struct TBF16S {}; // bfloat16 of (f_i - w_i), stochastically rounded
+struct TME16 {};  // 16-bit codes of the 19 moments of (f - w) (moment-encoded storage)
 
 template <class Tag> struct Codec;
 
@@ -119,6 +120,26 @@ template <> struct Codec<TBF16> {
 // Same format as TBF16; stores round stochastically (store_local).
 template <> struct Codec<TBF16S> : Codec<TBF16> {};
 
+// Moment codes: element k of a cell's 19 slots holds the 16-bit code of
+// moment k of its deviations; populations are decoded and encoded per cell
+// (load_local / store_local / pull below), never one element at a time.
+template <> struct Codec<TME16> {
+    typedef uint16_t raw;
+    typedef float C;
+};
+
+// Quantization constants of a moment-encoded lattice (device memory, FP32),
+// for deviation moments m'_k = sum_i M[k][i] (f_i - w_i) on the grid
+// m'_k = lo'_k + code_k step_k, lo'_k = lo_k - (M w)_k:
+//   decode  d_i = mag_i * sum_k sgn(Minv[i][k]) step_k code_k + L_i,
+//           L_i = sum_k Minv[i][k] lo'_k  (every non-zero of row i of M^-1
+//           has magnitude mag_i: 1 for the rest direction, 1/2 face, 1/4 edge);
+//   encode  code_k = floor((m'_k - lo'_k) sc_k + 1/2), clamped to [0, 65535],
+//           sc_k = 65535 / (hi_k - lo_k).
+struct MeQuant {
+    float step[19], L[19], lo[19], sc[19];
+};
+
 // Threads per block of the population kernels: one row segment of 256 cells
 // (4 or 8 bytes) or 512 cells (2 bytes) per direction and block.
 template <class Tag>
@@ -135,6 +156,14 @@ constexpr int step_block() {
 template <class Tag>
 constexpr int fused_block() {
     return std::is_same<Tag, TF32>::value ? 512 : step_block<Tag>();
+}
+
+// Wavefronts per SIMD the fused step is compiled for at least (1: no bound).
+// The moment-coded step (60-80 VGPRs) is held to four: one of its
+// instantiations was once allocated 142-166 (three wavefronts per SIMD).
+template <class Tag>
+constexpr int fused_min_waves() {
+    return std::is_same<Tag, TME16>::value ? 4 : 1;
 }
 
 // Blocks of the fused step (fused_block threads) resident per CU, 0 = as
@@ -193,6 +222,7 @@ template <class C> struct Params {
     // direction bounced at the z = 0 face (c_z > 0, [0]) and the z = nz - 1
     // face (c_z < 0, [1]), formed as the general path forms it
     C zterm[2][19];
+    const MeQuant* me;  // moment-encoded storage: its quantization constants, else null
 };
 
 // Fill the z-wall terms of p from its face velocities (host).
@@ -336,14 +366,153 @@ __device__ __forceinline__ void st(char* row, int i, uint32_t dir_bytes, uint32_
     __builtin_nontemporal_store(Codec<Tag>::enc(v, i), (R*)(seg_base<U>(row + (uint32_t)i * dir_bytes) + zoff));
 }
 
+// ---------------------------------------------------------------------------
+// Moment codes (storage TME16).  The moments are the reference variant's
+// (packages/lattice-boltzmann-d3q19-frontier-moment-encoded/src/lbm_me.cpp:
+// 83-90): monomials of c over the 19 velocities.
+// ---------------------------------------------------------------------------
+
+// Monomial k of velocity i (an integer: 0 or +-1).
+__host__ __device__ constexpr int me_mono(int k, int i) {
+    const int x = CX[i], y = CY[i], z = CZ[i];
+    switch (k) {
+        case 0: return 1;
+        case 1: return x;
+        case 2: return y;
+        case 3: return z;
+        case 4: return x * x;
+        case 5: return y * y;
+        case 6: return z * z;
+        case 7: return x * y;
+        case 8: return x * z;
+        case 9: return y * z;
+        case 10: return x * x * y;
+        case 11: return x * x * z;
+        case 12: return y * y * x;
+        case 13: return y * y * z;
+        case 14: return z * z * x;
+        case 15: return z * z * y;
+        case 16: return x * x * y * y;
+        case 17: return x * x * z * z;
+        default: return y * y * z * z;
+    }
+}
+
+// 4 M^-1 (exact integers): f_i = sum_k ME_MINV4[i][k] m_k / 4.  Solved from
+// the basis above; the host checks M M^-1 = I exactly (d3q19_moment_basis).
+constexpr int ME_MINV4[Q][Q] = {
+    {4, 0, 0, 0, -4, -4, -4, 0, 0, 0, 0, 0, 0, 0, 0, 0, 4, 4, 4},
+    {0, 2, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0, -2, 0, -2, 0, -2, -2, 0},
+    {0, -2, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 2, 0, 2, 0, -2, -2, 0},
+    {0, 0, 2, 0, 0, 2, 0, 0, 0, 0, -2, 0, 0, 0, 0, -2, -2, 0, -2},
+    {0, 0, -2, 0, 0, 2, 0, 0, 0, 0, 2, 0, 0, 0, 0, 2, -2, 0, -2},
+    {0, 0, 0, 2, 0, 0, 2, 0, 0, 0, 0, -2, 0, -2, 0, 0, 0, -2, -2},
+    {0, 0, 0, -2, 0, 0, 2, 0, 0, 0, 0, 2, 0, 2, 0, 0, 0, -2, -2},
+    {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 1, 0, 0, 0, 1, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, -1, 0, -1, 0, 0, 0, 1, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, -1, 0, 0, -1, 0, 1, 0, 0, 0, 1, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 1, 0, -1, 0, 0, 0, 1, 0, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, -1, 0, 0, -1, 0, 0, 1, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, -1, 0, 0, 1, 0, 0, 1, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 1, 0, 0, -1, 0, 0, 1, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 1, 0, 1, 0, 0, 1},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, -1, 0, -1, 0, 0, 1},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0, -1, 0, 1, 0, 0, 1},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, -1, 0, 0, 0, 1, 0, -1, 0, 0, 1},
+};
+
+// |Minv[i][k]| of the non-zeros of row i.
+__host__ __device__ constexpr float me_row_mag(int i) { return i == 0 ? 1.0f : (i <= 6 ? 0.5f : 0.25f); }
+
+// The quantization constants read through the constant address space:
+// invariant for the kernel, so the loads are scalar (one per wavefront).
+// Through the generic pointer the compiler could not rule out that the
+// step's stores alias them and loaded them per lane (20 dwordx4 vector loads
+// per thread; 256^3 step 31 against 47 GLUPS).
+typedef __attribute__((address_space(4))) const MeQuant MeQuantC;
+__device__ __forceinline__ MeQuantC* me_const(const MeQuant* q) { return (MeQuantC*)q; }
+
+// Deviation d_i of the cell whose slots start at `row` + zoff, from its codes
+// (the 3-5 codes of row i of M^-1; 91 over all 19 directions).  A pull
+// decodes each direction from the codes of its own source cell.
+template <bool U>
+__device__ __forceinline__ float me_pop(const char* row, int i, uint32_t DB, uint32_t zoff, const MeQuant* qg) {
+#pragma clang fp contract(off)
+    MeQuantC* q = me_const(qg);
+    float s = 0.0f;
+    bool first = true;
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        const int a = ME_MINV4[i][k];
+        if (a != 0) {
+            const float c = (float)*(const uint16_t*)(seg_base<U>(row + (uint32_t)k * DB) + zoff);
+            const float st = a > 0 ? q->step[k] : -q->step[k];
+            if (first) {
+                s = c * st;
+                first = false;
+            } else {
+                s = fmaf(c, st, s);
+            }
+        }
+    }
+    return fmaf(s, me_row_mag(i), q->L[i]);
+}
+
+// Even-degree monomials: the same value for c and -c.
+__host__ __device__ constexpr bool me_even(int k) { return k == 0 || (k >= 4 && k <= 9) || k >= 16; }
+
+// Encode the deviations d of a cell and store its 19 codes.  The moments are
+// summed over the opposite pairs (i, i + 1 = opp(i), i odd): the pair
+// contributes mono_k(c_i) (d_i + d_opp) to an even moment and
+// mono_k(c_i) (d_i - d_opp) to an odd one.
+template <bool U>
+__device__ __forceinline__ void me_store_cell(char* row, uint32_t DB, uint32_t zoff, const float d[Q],
+                                              const MeQuant* qg) {
+#pragma clang fp contract(off)
+    MeQuantC* q = me_const(qg);
+    float S[9], A[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+        S[j] = d[2 * j + 1] + d[2 * j + 2];
+        A[j] = d[2 * j + 1] - d[2 * j + 2];
+    }
+#pragma unroll
+    for (int k = 0; k < Q; ++k) {
+        float m = k == 0 ? d[0] : 0.0f;
+        bool first = k != 0;
+#pragma unroll
+        for (int j = 0; j < 9; ++j) {
+            const int c = me_mono(k, 2 * j + 1);
+            if (c != 0) {
+                const float t = me_even(k) ? S[j] : A[j];
+                if (first) {
+                    m = c > 0 ? t : -t;
+                    first = false;
+                } else {
+                    m = c > 0 ? m + t : m - t;
+                }
+            }
+        }
+        // nearest code, halves up; NaN and values below the range give 0
+        const float v = fminf(fmaxf(fmaf(m - q->lo[k], q->sc[k], 0.5f), 0.0f), 65535.0f);
+        __builtin_nontemporal_store((uint16_t)(uint32_t)v, (uint16_t*)(seg_base<U>(row + (uint32_t)k * DB) + zoff));
+    }
+}
+
 // In-place (unstreamed) populations of cell (x, y, z).
 template <class Tag, bool U = false>
 __device__ __forceinline__ void load_local(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                            uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
     const char* row = row_of(in, p, x, y);
+    if constexpr (std::is_same<Tag, TME16>::value) {
 #pragma unroll
-    for (int i = 0; i < Q; ++i) d[i] = ld<Tag, false, U>(row, i, p.dir_bytes, z * ES);
+        for (int i = 0; i < Q; ++i) d[i] = me_pop<U>(row, i, p.dir_bytes, z * ES, p.me);
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i) d[i] = ld<Tag, false, U>(row, i, p.dir_bytes, z * ES);
+    }
 }
 
 // 32-bit integer hash (two xorshift-multiply rounds).
@@ -387,10 +556,14 @@ __device__ __forceinline__ void store_local(void* out, const Params<typename Cod
     // an opaque 32-bit lane offset: with the forced collision in the same
     // kernel LLVM otherwise widened it to 64 bits and added it per store
     if constexpr (U) asm volatile("" : "+v"(zoff));
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        me_store_cell<U>(row, p.dir_bytes, zoff, d, p.me);
+    } else {
 #pragma unroll
-    for (int i = 0; i < Q; ++i)
-        __builtin_nontemporal_store(raw_of<Tag>(p, x, y, z, d[i], i),
-                                    (R*)(seg_base<U>(row + (uint32_t)i * p.dir_bytes) + zoff));
+        for (int i = 0; i < Q; ++i)
+            __builtin_nontemporal_store(raw_of<Tag>(p, x, y, z, d[i], i),
+                                        (R*)(seg_base<U>(row + (uint32_t)i * p.dir_bytes) + zoff));
+    }
 }
 
 // Reference y-wall treatment (reference/bgk.py:190-215) applied in registers
@@ -679,10 +852,60 @@ __device__ __forceinline__ void pull_halfway_general(const void* in, const Param
     }
 }
 
+// Pull of a moment-encoded lattice: each direction decoded from the codes of
+// its source cell (me_pop); a bounce-back (half-way wall face or solid
+// source) decodes the cell's own opposite direction instead, in a branch
+// that is wavefront-uniform away from walls and solids.  The reference
+// y-walls act on the pulled deviations (ywall_swap).
+template <int BC, bool U>
+__device__ __forceinline__ void me_pull(const void* in, const Params<float>& p, uint32_t x, uint32_t y, uint32_t z,
+                                        float d[Q]) {
+#pragma clang fp contract(off)
+    D3Q19_PULL_SOURCES(p, x, y, z);
+    const uint32_t DB = p.dir_bytes;
+    uint32_t zo[3] = {zs[0] * 2u, zs[1] * 2u, zs[2] * 2u};
+    if constexpr (U) asm volatile("" : "+v"(zo[0]), "+v"(zo[1]), "+v"(zo[2]));
+    if constexpr (BC == BC_HALFWAY) {
+        const uint32_t wm = p.wall_mask;
+        const char* self = row_of(in, p, x, y);
+        bool solid_src[Q];
+#pragma unroll
+        for (int i = 0; i < Q; ++i) solid_src[i] = false;
+        if (p.solid) {
+#pragma unroll
+            for (int i = 1; i < Q; ++i)
+                solid_src[i] = p.solid[(xs[1 - CX[i]] * p.ny + ys[1 - CY[i]]) * p.nz + zs[1 - CZ[i]]] != 0;
+        }
+#pragma unroll
+        for (int i = 0; i < Q; ++i) {
+            const int f = halfway_face<float>(i, wm, x, y, z, p);
+            float v;
+            if (f >= 0 || solid_src[i])
+                v = me_pop<U>(self, OPP[i], DB, zo[1], p.me);
+            else
+                v = me_pop<U>(row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]), i, DB, zo[1 - CZ[i]], p.me);
+            if (f >= 0) v = v + 6.0f * wgt<float>(i) * cdot<float>(i, p.face_u[f][0], p.face_u[f][1], p.face_u[f][2]);
+            d[i] = v;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < Q; ++i)
+            d[i] = me_pop<U>(row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]), i, DB, zo[1 - CZ[i]], p.me);
+        // Every decoded value complete here: otherwise the forced periodic
+        // step kept loads in flight across the force-mode branches of the
+        // collision and spilled 23 VGPRs to scratch at 128.
+#pragma unroll
+        for (int i = 0; i < Q; ++i) asm volatile("" : "+v"(d[i]));
+        if constexpr (BC == BC_YWALLS) ywall_swap<float>(p, y, d);
+    }
+}
+
 template <class Tag, int BC, bool NT = false, bool U = false>
 __device__ __forceinline__ void pull(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                      uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
-    if constexpr (BC == BC_HALFWAY) {
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        me_pull<BC, U>(in, p, x, y, z, d);
+    } else if constexpr (BC == BC_HALFWAY) {
         constexpr bool short_raw = sizeof(typename Codec<Tag>::raw) == 2;
         if (p.solid || (!short_raw && halfway_edge_row(p, x, y)))
             pull_halfway_general<Tag, NT>(in, p, x, y, z, d);

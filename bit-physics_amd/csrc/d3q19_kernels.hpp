@@ -25,7 +25,7 @@ __device__ __forceinline__ void step_moments_out(float* rho_out, float* u_out, u
 // (optionally) rho and u of the reference state f^n from the same pass.
 // ---------------------------------------------------------------------------
 template <class Tag, int BC, bool FORCED, bool ROWS = true>
-static __global__ void __launch_bounds__(fused_block<Tag>())
+static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag>())
     k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
@@ -39,7 +39,7 @@ static __global__ void __launch_bounds__(fused_block<Tag>())
     constexpr bool U = ROWS && sizeof(typename Codec<Tag>::raw) == 2;
     C d[Q];
     C rho = C(1), ux = C(0), uy = C(0), uz = C(0);
-    if constexpr (BC == BC_HALFWAY && sizeof(typename Codec<Tag>::raw) == 2) {
+    if constexpr (BC == BC_HALFWAY && sizeof(typename Codec<Tag>::raw) == 2 && !std::is_same<Tag, TME16>::value) {
         // 16-bit storage without solids: interior rows and edge rows each
         // with a collision and store of their own, compiled apart from each
         // other and from the general path below (fewer live scalar registers

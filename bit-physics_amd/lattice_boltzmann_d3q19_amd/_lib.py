@@ -36,7 +36,7 @@ EXPORTED = (
     "d3q19_adjoint_destroy",
     "d3q19_adjoint_set_stream", "d3q19_adjoint_forward", "d3q19_adjoint_backward",
     "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
-    "d3q19_moment_basis", "d3q19_set_moment_ranges", "d3q19_moment_range", "d3q19_health",
+    "d3q19_moment_basis", "d3q19_set_moment_ranges", "d3q19_moment_range", "d3q19_moment_range_stored", "d3q19_health",
     "d3q19_set_step_count", "d3q19_state_bytes", "d3q19_read_state", "d3q19_write_state",
     "d3q19_feq", "d3q19_shader_clock", "d3q19_read_moments_noalloc",
 )
@@ -147,6 +147,7 @@ def load() -> ctypes.CDLL:
         "d3q19_moment_basis": (ctypes.c_int, [P, P]),
         "d3q19_set_moment_ranges": (ctypes.c_int, [P, P, P]),
         "d3q19_moment_range": (ctypes.c_int, [P, P, P]),
+        "d3q19_moment_range_stored": (ctypes.c_int, [P, P, P]),
         "d3q19_health": (ctypes.c_int, [P, P]),
         "d3q19_set_step_count": (ctypes.c_int, [P, ctypes.c_int64]),
         "d3q19_state_bytes": (ctypes.c_size_t, [P]),

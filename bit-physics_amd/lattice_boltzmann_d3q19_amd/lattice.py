@@ -61,8 +61,10 @@ class Lattice:
     tau : BGK relaxation time (> 0.5)
     storage : "f32" (default) | "f64" | "f16" | "bf16" | "bf16sr" | "me16" | "me16ref" --
         population storage; 16-bit formats store f_i - w_i ("bf16sr":
-        stochastically rounded) or 16-bit moment codes ("me16"); collision
-        stays FP32.  "me16ref": the same moment codes stepped in the
+        stochastically rounded) or the 16-bit codes of the 19 moments of
+        f - w ("me16": the stored post-collision state, one fused pass per
+        step; ranges from moment_range(stored=True) of an FP64 run);
+        collision stays FP32.  "me16ref": moment codes of f^n stepped in the
         moment-encoded reference variant's FP64 arithmetic (periodic or
         y-walls; no reductions).
     boundary : "periodic" | "ywalls" (the reference's apply_bounce_back_y_walls)
@@ -393,8 +395,8 @@ class Lattice:
 
     # ---- moment-encoded storage ----------------------------------------
     def set_moment_ranges(self, lo, hi):
-        """Quantization ranges [lo_k, hi_k] of the 19 moments of f ("me16"
-        storage; before the first state is loaded)."""
+        """Quantization ranges [lo_k, hi_k] of the 19 moments of f ("me16" /
+        "me16ref" storage; before the first state is loaded)."""
         lo = np.ascontiguousarray(lo, dtype=np.float64)
         hi = np.ascontiguousarray(hi, dtype=np.float64)
         if lo.shape != (19,) or hi.shape != (19,):
@@ -403,13 +405,16 @@ class Lattice:
         self._me_ranges = (lo.copy(), hi.copy())
         return self
 
-    def moment_range(self):
+    def moment_range(self, stored: bool = False):
         """(lo, hi): FP64 minimum and maximum over all cells of the 19 moments
-        m = M f of the current state."""
+        m = M f of the current state f^n, or with ``stored`` of the state as
+        the lattice stores it (the post-collision populations of the last
+        step; f^n after a load) -- the states an "me16" lattice encodes."""
         self._sync_stream()
         lo = np.empty(19, np.float64)
         hi = np.empty(19, np.float64)
-        call("d3q19_moment_range", self._h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
+        fn = "d3q19_moment_range_stored" if stored else "d3q19_moment_range"
+        call(fn, self._h, ctypes.c_void_p(lo.ctypes.data), ctypes.c_void_p(hi.ctypes.data))
         return lo, hi
 
     def enstrophy(self) -> float:

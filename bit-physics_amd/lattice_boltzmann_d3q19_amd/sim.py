@@ -185,16 +185,17 @@ def sim_runner_moment_encoded(seed: int, out_dir, steps: int = CANONICAL_POISEUI
     """Canonical Poiseuille on 16-bit moment-encoded storage, every step
     captured (ME/src/lbm_me_capture_main.cpp, lbm_me.cpp:395-486): ranges
     calibrated over the whole horizon (margin 0.25), frame 0 is the encoded
-    initial state."""
+    initial state.  The codes are stepped in the variant's own FP64
+    arithmetic ("me16ref"), so the frames are the variant's."""
     from .moment_encoded import RANGE_MARGIN, WARMUP_STEPS, moment_encoded_lattice
 
     nx, ny, nz = CANONICAL_POISEUILLE_NX, CANONICAL_POISEUILLE_NY, CANONICAL_NZ
     t0 = time.perf_counter()
     lat = moment_encoded_lattice((nx, ny, nz), CANONICAL_TAU, steps, boundary="ywalls",
-                                 force=(CANONICAL_POISEUILLE_FORCE_X, 0.0, 0.0))
+                                 force=(CANONICAL_POISEUILLE_FORCE_X, 0.0, 0.0), reference_arithmetic=True)
     params = {"tau": CANONICAL_TAU, "force_x_lattice": CANONICAL_POISEUILLE_FORCE_X, "quantize_bits": 16.0,
               "range_margin": RANGE_MARGIN, "warmup_steps": float(WARMUP_STEPS)}
-    m = _manifest(CANONICAL_DESCRIPTOR_POISEUILLE, "test", (nx, ny, nz), seed, params, steps, 1, "f32")
+    m = _manifest(CANONICAL_DESCRIPTOR_POISEUILLE, "test", (nx, ny, nz), seed, params, steps, 1, "f64")
     m.sim["variant"] = "frontier-moment-encoded-16bit"
 
     def frames():

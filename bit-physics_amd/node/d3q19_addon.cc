@@ -11,7 +11,7 @@
 //   lat.readMoments();                   // {rho: Float64Array, u: Float64Array}
 //   lat.diagnostics();                   // {mass, momentum: [3], kineticEnergy}
 //   lat.setMomentRanges(lo, hi);         // "me16" storage: moment quantization ranges
-//   lat.momentRange();                   // {lo, hi} of the 19 moments of the state
+//   lat.momentRange(stored?);            // {lo, hi} of the 19 moments of f^n (stored: of the stored state)
 //   lat.health();                        // {checkHealth, nonfiniteCells, rhoMin, rhoMax, uMaxLat}
 //   lat.close();
 //
@@ -326,17 +326,31 @@ napi_value SetMomentRanges(napi_env env, napi_callback_info info) {
     return nullptr;
 }
 
-// momentRange(): {lo, hi} of the 19 moments of the current state.
+// momentRange(stored = false): {lo, hi} of the 19 moments of the current
+// state f^n, or (stored) of the state as the lattice stores it.
 napi_value MomentRange(napi_env env, napi_callback_info info) {
-    size_t argc = 0;
-    Wrapped* w = unwrap(env, info, &argc, nullptr);
+    size_t argc = 1;
+    napi_value argv[1];
+    Wrapped* w = unwrap(env, info, &argc, argv);
     if (!w) return nullptr;
+    bool stored = false;
+    if (argc >= 1) {
+        napi_valuetype t;
+        NAPI_CALL(env, napi_typeof(env, argv[0], &t));
+        if (t == napi_boolean) {
+            NAPI_CALL(env, napi_get_value_bool(env, argv[0], &stored));
+        } else if (t != napi_undefined) {
+            napi_throw_type_error(env, nullptr, "momentRange(stored): stored must be a boolean");
+            return nullptr;
+        }
+    }
     void *plo = nullptr, *phi = nullptr;
     napi_value lo = make_typed(env, D3Q19_F64, 19, &plo);
     napi_value hi = make_typed(env, D3Q19_F64, 19, &phi);
     if (!lo || !hi) return nullptr;
-    int st = d3q19_moment_range(w->lat, (double*)plo, (double*)phi);
-    if (st) return throw_status(env, "d3q19_moment_range", st);
+    int st = stored ? d3q19_moment_range_stored(w->lat, (double*)plo, (double*)phi)
+                    : d3q19_moment_range(w->lat, (double*)plo, (double*)phi);
+    if (st) return throw_status(env, stored ? "d3q19_moment_range_stored" : "d3q19_moment_range", st);
     napi_value obj;
     NAPI_CALL(env, napi_create_object(env, &obj));
     NAPI_CALL(env, napi_set_named_property(env, obj, "lo", lo));

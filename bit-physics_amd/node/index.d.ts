@@ -54,7 +54,8 @@ export declare class Lattice {
   // 'me16' / 'me16ref' storage: quantization ranges of the 19 moments of f, before the state is loaded
   setMomentRanges(lo: Float64Array, hi: Float64Array): void;
   // minimum and maximum over all cells of the 19 moments of the current state
-  momentRange(): { lo: Float64Array; hi: Float64Array };
+  // stored: of the state as the lattice stores it (post-collision, what 'me16' encodes)
+  momentRange(stored?: boolean): { lo: Float64Array; hi: Float64Array };
   health(): Health;
   close(): void;
 }

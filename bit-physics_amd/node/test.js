@@ -69,15 +69,16 @@ const out = { version: version() };
   lat.close();
 }
 {
-  // moment-encoded storage: ranges from the FP64 trajectory, then the quantized run
+  // moment-encoded storage: ranges from the FP64 trajectory's stored states, then the quantized run
   const nx = 64, ny = 32, nz = 3, n = nx * ny * nz, steps = 1000;
   const cfg = { nx, ny, nz, tau: 0.7, boundary: 'ywalls', force: [1e-5, 0, 0] };
   const ref = new Lattice({ ...cfg, storage: 'f64' });
   ref.setEquilibrium(null, new Float64Array(3 * n));
-  let { lo, hi } = ref.momentRange();
+  let { lo, hi } = ref.momentRange(true);
+  assert.throws(() => ref.momentRange('yes'), TypeError, 'momentRange takes a boolean');
   for (let s = 0; s < steps; ++s) {
     ref.step(1);
-    const r = ref.momentRange();
+    const r = ref.momentRange(true);
     for (let k = 0; k < 19; ++k) { lo[k] = Math.min(lo[k], r.lo[k]); hi[k] = Math.max(hi[k], r.hi[k]); }
   }
   const uRef = ref.readMoments().u;

@@ -116,9 +116,10 @@ LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_man
 // reference_fp64 (what run_lbm above uses) steps the codes as the reference
 // variant's shaders do, decoding to FP64 populations, colliding (division-form
 // equilibrium), streaming with the y-wall bounce-back and encoding the result
-// (D3Q19_ME16_REF); engine_fp32 uses the engine's two-pass FP32 moment-coded
-// step (D3Q19_ME16: deviation moments, FP32 collision), faster and within the
-// lbm-quantized tolerance of the FP64 one.
+// (D3Q19_ME16_REF); engine_fp32 uses the engine's fused FP32 moment-coded
+// step (D3Q19_ME16: codes of the stored post-collision state, ranges
+// calibrated over those states and reported in ranges_used), faster and
+// within the lbm-quantized tolerance of the FP64 run.
 enum class QuantArithmetic { reference_fp64, engine_fp32 };
 LbmResult run_lbm(const LbmConfig& cfg, const std::filesystem::path* capture_manifest, QuantArithmetic arithmetic);
 

@@ -13,7 +13,8 @@
  *  - dtype codes: D3Q19_F64, D3Q19_F32 for arrays crossing the ABI; the
  *    lattice storage type may also be D3Q19_F16 / D3Q19_BF16 / D3Q19_BF16SR
  *    (populations kept on the GPU as f_i - w_i, collision in FP32) or
- *    D3Q19_ME16 (the 19 monomial moments quantized to 16 bits, collision in FP32).
+ *    D3Q19_ME16 (the 19 monomial moments of f - w quantized to 16 bits,
+ *    collision in FP32).
  *  - on_device != 0: the pointer is device memory on the lattice's device;
  *    otherwise host memory (the library stages it).
  *  - stream: a hipStream_t passed as void*; NULL is HIP's default stream.
@@ -110,7 +111,7 @@ int d3q19_set_stream(d3q19_lattice* lat, void* stream);
 int d3q19_set_populations(d3q19_lattice* lat, const void* f, int32_t dtype, int32_t on_device);
 /* Set the state to feq_field(rho, u) (NP/reference/equilibrium.py:71-98,
    NP/sim.py:281-285).  rho may be NULL for rho = 1.  One pass, no
-   (19,Nx,Ny,Nz) staging (D3Q19_ME16: through populations). */
+   (19,Nx,Ny,Nz) staging. */
 int d3q19_set_equilibrium(d3q19_lattice* lat, const void* rho, const void* u, int32_t dtype,
                           int32_t on_device);
 /* Per-cell body force (3,Nx,Ny,Nz) for D3Q19_FORCE_FIELD (bgk.py:45-49 force_lattice). */
@@ -174,23 +175,36 @@ int d3q19_write_state(d3q19_lattice* lat, const void* src, int32_t local, int64_
 size_t d3q19_device_bytes(const d3q19_lattice* lat);
 
 /* ---- 16-bit moment-encoded storage (D3Q19_ME16) -------------------------
- * Replaces the frontier-moment-encoded variant's codec
+ * The frontier-moment-encoded variant's codec
  * (packages/lattice-boltzmann-d3q19-frontier-moment-encoded: src/lbm_me.cpp:83-150
  * build_moment_basis, :222-253 calibrate_ranges, shaders/lbm_encode.comp,
- * shaders/lbm_decode.comp): the lattice keeps per cell the 19 moments m = M f
- * over the velocity-set monomials (rows 0..3: density and momentum), each as
- * code = round((m - lo) / (hi - lo) * 65535) clamped; f = M^-1 m on use.  The
- * codes hold the pre-collision state f^n; a step decodes, collides (FP32),
- * pulls, applies the boundary and re-encodes.  Device memory: 38 B of codes +
- * 76 B of FP32 work lattice per cell. */
+ * shaders/lbm_decode.comp) as a population storage of the engine: per cell
+ * the 19 moments m = M f over the velocity-set monomials (rows 0..3: density
+ * and momentum), each as code = nearest((m - lo) / (hi - lo) * 65535),
+ * clamped; f = M^-1 m on use.  The codes hold what every storage holds: the
+ * post-collision populations of the last step (f^n in place after a load).
+ * One fused pass per step: each pulled population is decoded from its
+ * source cell's codes (row i of M^-1, 3-5 codes), the cell collides in FP32
+ * and its 19 codes are stored.  Device memory: 2 x 38 B per cell.  The
+ * ranges must cover the moments of every stored state: calibrate them with
+ * d3q19_moment_range_stored.  Storage D3Q19_ME16_REF (above) keeps the
+ * variant's own quantization point (f^n) and FP64 arithmetic instead. */
 /* M (moments = M f, row-major 19 x 19, entries 0/+-1) and 4 M^-1 (integers). */
 int d3q19_moment_basis(int32_t m[361], int32_t minv4[361]);
-/* Quantization ranges of the 19 moments of f (not of f - w); ME16 lattices
- * only, before the first set_populations / set_equilibrium. */
+/* Quantization ranges of the 19 moments of f (not of f - w); ME16 and
+ * ME16_REF lattices only, before the first set_populations /
+ * set_equilibrium / write_state. */
 int d3q19_set_moment_ranges(d3q19_lattice* lat, const double lo[19], const double hi[19]);
 /* FP64 minimum and maximum over all cells of the 19 moments of the current
  * state f^n (any storage): one frame of the range calibration. */
 int d3q19_moment_range(d3q19_lattice* lat, double lo[19], double hi[19]);
+/* The same over the stored state as it lies in the lattice's buffer: the
+ * post-collision populations of the last step, or the loaded f^n before the
+ * first step (any storage but ME16_REF).  A D3Q19_ME16 lattice encodes
+ * exactly these states, so an FP64 run's ranges from this call, frame 0 ..
+ * T, cover a T-step D3Q19_ME16 run of the same configuration.  No reference
+ * counterpart: the variant quantizes f^n. */
+int d3q19_moment_range_stored(d3q19_lattice* lat, double lo[19], double hi[19]);
 
 /* ---- stateless drop-ins (NP/reference/__init__.py:43-71) ---------------
  * Device pointers, dtype D3Q19_F64 or D3Q19_F32, (19,Nx,Ny,Nz) layout,

@@ -113,7 +113,7 @@ def test_capture_equals_python_lattice(dump, mode):
     assert cap.metadata["sim_variant"] == m["sim"]["variant"] and int(cap.metadata["seed"]) == 42
     if mode == "me16":
         plo, phi = calibrate_moment_ranges(SHAPE, TAU, STEPS, populations=rest(SHAPE), boundary="ywalls",
-                                           force=(FX, 0.0, 0.0))
+                                           force=(FX, 0.0, 0.0), reference_arithmetic=True)
         assert np.array_equal(plo, lo) and np.array_equal(phi, hi)
         # run_lbm steps the codes in the reference variant's FP64 arithmetic
         frames, f = python_run(amd, "me16ref", lo, hi)
