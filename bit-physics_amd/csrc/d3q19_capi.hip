@@ -228,6 +228,7 @@ int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
 template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
     typedef typename Codec<Tag>::C C;
+    L->steps = 0;  // before make_params: the stochastic rounding of the load is keyed on it
     const Params<C> p = make_params<C>(L);
     const StepGrid sg = step_grid<Tag>(L);
     if (L->rows)
@@ -245,6 +246,7 @@ int launch_init(d3q19_lattice* L, const In* f_dev) {
 template <class Tag, class In>
 int launch_init_eq(d3q19_lattice* L, const In* rho_dev, const In* u_dev) {
     typedef typename Codec<Tag>::C C;
+    L->steps = 0;  // before make_params: the stochastic rounding of the load is keyed on it
     const Params<C> p = make_params<C>(L);
     const StepGrid sg = step_grid<Tag>(L);
     if (L->rows)
@@ -514,24 +516,27 @@ extern "C" int d3q19_destroy(d3q19_lattice* L);
 namespace {
 
 // Quantization constants of a D3Q19_ME16 lattice from its ranges: the
-// deviation-moment grid lo'_k = lo_k - (M w)_k, step_k = (hi_k - lo_k) / 65535
-// and the decode offsets L_i = sum_k Minv[i][k] lo'_k, formed in FP64.
+// deviation-moment grid lo'_k = fp32(lo_k - (M w)_k), step_k = the power of
+// two at or above (hi_k - lo_k) / 65535 (so the 65536 codes cover [lo, hi]
+// and 1 / step_k is exact), and the decode offsets sum_k Minv[i][k] lo'_k
+// formed in FP64 and kept as a sum of two floats.
 MeQuant me_quant(const double lo[19], const double hi[19]) {
     MeQuant q;
     double lod[Q];
     for (int k = 0; k < Q; ++k) {
         double mw = 0.0;  // moment k of the rest state
         for (int i = 0; i < Q; ++i) mw += me_mono(k, i) * wgt<double>(i);
-        const double span = hi[k] - lo[k];
-        lod[k] = lo[k] - mw;
-        q.step[k] = (float)(span / 65535.0);
+        const double st = std::exp2(std::ceil(std::log2((hi[k] - lo[k]) / 65535.0)));
+        lod[k] = (double)(float)(lo[k] - mw);
+        q.step[k] = (float)st;
         q.lo[k] = (float)lod[k];
-        q.sc[k] = (float)(65535.0 / span);
+        q.sc[k] = (float)(1.0 / st);
     }
     for (int i = 0; i < Q; ++i) {
         double l = 0.0;
         for (int k = 0; k < Q; ++k) l += 0.25 * ME_MINV4[i][k] * lod[k];
         q.L[i] = (float)l;
+        q.Ll[i] = (float)(l - (double)q.L[i]);
     }
     return q;
 }
