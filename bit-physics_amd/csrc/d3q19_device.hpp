@@ -676,14 +676,17 @@ __device__ __forceinline__ bool halfway_edge_row(const Params<C>& p, uint32_t x,
            ((wm & 8u) && y + 1 == p.ny);
 }
 
-// z-wall bounce values, loaded raw.  A lane at z = 0 bounces, for each i with
-// c_z > 0, its own opp(i); a lane at z = nz - 1, for each i with c_z < 0,
-// its own opp(i).  The address is the same in every wall lane of a
-// wavefront (one row, fixed z), so on the row grid it is a scalar base.
+// z-wall bounce values of 16-bit storage, loaded raw.  A lane at z = 0
+// bounces, for each i with c_z > 0, its own opp(i); a lane at z = nz - 1, for
+// each i with c_z < 0, its own opp(i).  The address is the same in every wall
+// lane of a wavefront (one row, fixed z), so on the row grid it is a scalar
+// base.  Each value is held zero-extended in a 32-bit register of its own:
+// held as 16-bit values, the compiler packed two per register right after
+// the loads, and the packing waited for them before the pull was issued (a
+// full memory round trip more for every wavefront holding a wall lane).
 template <class Tag, bool U>
 __device__ __forceinline__ void zwall_fetch(const char* self, const Params<typename Codec<Tag>::C>& p, bool zlo,
-                                            bool zhi, typename Codec<Tag>::raw rlo[Q],
-                                            typename Codec<Tag>::raw rhi[Q]) {
+                                            bool zhi, uint32_t rlo[Q], uint32_t rhi[Q]) {
     typedef typename Codec<Tag>::raw R;
     const uint32_t DB = p.dir_bytes;
     if (zlo) {
@@ -699,23 +702,35 @@ __device__ __forceinline__ void zwall_fetch(const char* self, const Params<typen
     }
 }
 
+// Pin the fetched z-wall values in their registers once the pull is issued:
+// the decode of a value loaded in a branch was otherwise folded into that
+// branch (next to the load), where it waited for the load before the pull.
+__device__ __forceinline__ void zwall_pin(uint32_t rlo[Q], uint32_t rhi[Q]) {
+#pragma unroll
+    for (int i = 0; i < Q; ++i) {
+        if (CZ[i] > 0) asm volatile("" : "+v"(rlo[i]));
+        if (CZ[i] < 0) asm volatile("" : "+v"(rhi[i]));
+    }
+}
+
 // The z-wall lanes' bounced directions: opp(i) decodes with index i (same
 // weight class), plus the face's precomputed moving-wall term.  Applied
 // after any x/y bounce: at an edge the z face is the one that counts.
 template <class Tag>
 __device__ __forceinline__ void zwall_apply(const Params<typename Codec<Tag>::C>& p, bool zlo, bool zhi,
-                                            const typename Codec<Tag>::raw rlo[Q],
-                                            const typename Codec<Tag>::raw rhi[Q], typename Codec<Tag>::C d[Q]) {
+                                            const uint32_t rlo[Q], const uint32_t rhi[Q],
+                                            typename Codec<Tag>::C d[Q]) {
 #pragma clang fp contract(off)
+    typedef typename Codec<Tag>::raw R;
     if (zlo) {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            if (CZ[i] > 0) d[i] = Codec<Tag>::dec(rlo[i], i) + p.zterm[0][i];
+            if (CZ[i] > 0) d[i] = Codec<Tag>::dec((R)rlo[i], i) + p.zterm[0][i];
     }
     if (zhi) {
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            if (CZ[i] < 0) d[i] = Codec<Tag>::dec(rhi[i], i) + p.zterm[1][i];
+            if (CZ[i] < 0) d[i] = Codec<Tag>::dec((R)rhi[i], i) + p.zterm[1][i];
     }
 }
 
@@ -737,9 +752,10 @@ __device__ __forceinline__ void pull_halfway_fast(const void* in, const Params<t
     const char* self = row_of(in, p, x, y);
     const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
     if constexpr (sizeof(R) == 2) {
-        R rlo[Q], rhi[Q];
+        uint32_t rlo[Q], rhi[Q];
         zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
         pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
+        zwall_pin(rlo, rhi);
         zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
     } else {
         pull_periodic<Tag, NT, U>(in, p, z, xs, ys, zs, d);
@@ -773,7 +789,7 @@ __device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<t
     const uint32_t wm = p.wall_mask;
     const char* self = row_of(in, p, x, y);
     const bool zlo = z == 0 && (wm & 16u), zhi = z + 1 == p.nz && (wm & 32u);
-    R rlo[Q], rhi[Q];
+    uint32_t rlo[Q], rhi[Q];
     zwall_fetch<Tag, U>(self, p, zlo, zhi, rlo, rhi);
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
@@ -792,6 +808,7 @@ __device__ __forceinline__ void pull_halfway_edge(const void* in, const Params<t
             d[i] = v;
         }
     }
+    zwall_pin(rlo, rhi);
     zwall_apply<Tag>(p, zlo, zhi, rlo, rhi, d);
 }
 
