@@ -52,15 +52,16 @@ class ShaderClock:
         return round(mhz.value, 1)
 
 
-def settle(torch, lat, settle_ms, chunk=10):
-    """Run steps of `lat` until settle_ms of device time have passed; returns
-    the number of steps.  At the start of a heavy launch sequence after host
-    idle time the MI355X shader clock dips (2400 -> ~1700 MHz) for some tens
-    of milliseconds and recovers; a VALU-heavier step (16-bit storage) slows
-    by up to 25% inside the dip, a memory-bound one barely moves."""
+def settle(torch, lat, settle_ms, chunk=10, max_steps=None):
+    """Run steps of `lat` until settle_ms of device time have passed (or
+    max_steps steps); returns the number of steps.  At the start of a heavy
+    launch sequence after host idle time the MI355X shader clock dips
+    (2400 -> ~1700 MHz) for some tens of milliseconds and recovers; a
+    VALU-heavier step (16-bit storage, moment codes) slows by up to 25% inside
+    the dip, a memory-bound one barely moves."""
     stream = torch.cuda.current_stream()
     n, ms = 0, 0.0
-    while ms < settle_ms:
+    while ms < settle_ms and (max_steps is None or n + chunk <= max_steps):
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         lat.step(chunk)
@@ -71,18 +72,27 @@ def settle(torch, lat, settle_ms, chunk=10):
     return n
 
 
-def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup, settle_ms=0.0, clock=None):
+# Untimed steps at most before the moment-encoded leg's settled window: its
+# code ranges are calibrated over this many steps more than warm-up + timed.
+ME16_SETTLE_CAP = 1000
+
+
+def time_lattice(amd, torch, shape, tau, storage, rho0, u0, steps, warmup, settle_ms=0.0, clock=None,
+                 me_ranges=None):
     """Time `steps` steps after `warmup` untimed ones (plus, with settle_ms > 0,
-    untimed steps for that much device time before the warm-up)."""
+    untimed steps for that much device time before the warm-up).  A
+    moment-encoded ("me16") lattice takes its code ranges from me_ranges
+    (calibrated over warm-up + timed steps if not given)."""
     stream = torch.cuda.current_stream()
     settle_steps = 0
     if storage == "me16":
-        # ranges of the moments over the whole run (warm-up + timed steps)
         from lattice_boltzmann_d3q19_amd.moment_encoded import moment_encoded_lattice
 
+        if settle_ms > 0 and me_ranges is None:
+            raise ValueError("a settled moment-encoded window needs ranges calibrated past the settle steps")
+        lat = moment_encoded_lattice(shape, tau, warmup + steps, ranges=me_ranges, rho=rho0, u=u0)
         if settle_ms > 0:
-            raise ValueError("the moment-encoded leg runs without a settle phase")
-        lat = moment_encoded_lattice(shape, tau, warmup + steps, rho=rho0, u=u0)
+            settle_steps = settle(torch, lat, settle_ms, max_steps=ME16_SETTLE_CAP)
     else:
         lat = amd.Lattice(shape, tau, storage=storage)
         lat.set_equilibrium(rho0, u0)
@@ -552,7 +562,7 @@ def main() -> int:
     ap.add_argument("--u0", type=float, default=0.04)
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--settle-ms", type=float, default=200.0,
-                    help="untimed device time before the FP16 leg's settled window")
+                    help="untimed device time before the FP16 and moment-encoded legs' settled windows")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16", action="store_true")
     ap.add_argument("--no-me16", action="store_true")
@@ -690,8 +700,21 @@ def main() -> int:
         }
 
     if not args.no_me16:
-        ms_me, _, diag_me, _ = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps, args.warmup)
+        # Moment codes: ranges calibrated once (FP64 run on the GPU) over the
+        # longest horizon either window runs; then, as for FP16, K steps after
+        # W warm-up steps (the first window, whose kinetic energy is compared
+        # with FP32's at the same step) and K steps after `settle_ms` of
+        # untimed steps past the shader-clock dip (the rate reported).
+        from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+        me_ranges = calibrate_moment_ranges(shape, args.tau, args.warmup + args.steps + ME16_SETTLE_CAP,
+                                            rho=rho0, u=u0)
+        ms_mef, _, diag_me, info_mef = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps,
+                                                    args.warmup, clock=clock, me_ranges=me_ranges)
+        ms_me, _, _, info_me = time_lattice(amd, torch, shape, args.tau, "me16", rho0, u0, args.steps, args.warmup,
+                                            settle_ms=args.settle_ms, clock=clock, me_ranges=me_ranges)
         glups_me = ncell * args.steps / (ms_me * 1e-3) / 1e9
+        glups_mef = ncell * args.steps / (ms_mef * 1e-3) / 1e9
         ach_me = bytes_per_cell("me16") * ncell / (ms_me / args.steps * 1e-3) / 1e9
         result["me16_store"] = {
             "value": round(glups_me, 4),
@@ -700,6 +723,12 @@ def main() -> int:
             "algorithmic_bytes_per_cell": bytes_per_cell("me16"),
             "achieved_GBs": round(ach_me, 1),
             "frac": round(ach_me / HBM_PEAK_GBS, 4),
+            "settle_ms": args.settle_ms,
+            "settle_steps": info_me["settle_steps"],
+            "sclk_MHz": [info_me["sclk_MHz_before"], info_me["sclk_MHz_after"]],
+            "first_window": {"value": round(glups_mef, 4), "ratio_vs_fp32": round(glups_mef / glups, 4),
+                             "sclk_MHz": [info_mef["sclk_MHz_before"], info_mef["sclk_MHz_after"]],
+                             "note": "K steps after only W warm-up steps, as the FP32 leg"},
             "ke_rel_diff_vs_fp32": abs(diag_me["kinetic_energy"] - diag["kinetic_energy"]) / diag["kinetic_energy"],
         }
 
