@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -98,6 +99,7 @@ struct d3q19_lattice {
     double me_lo[Q] = {}, me_hi[Q] = {};
     double* mr_partials = nullptr;  // moment-range partials and result
     double* mr_out = nullptr;
+    uint32_t chunk_override = 0;  // plane chunks per fused step, 0 = by step_chunks (D3Q19_STEP_CHUNKS)
     int64_t steps = 0;
     size_t bytes = 0;
     uint32_t lds_per_cu = 0;  // LDS bytes per CU of the device, 0 if unknown
@@ -177,8 +179,8 @@ StepGrid step_grid(const d3q19_lattice* L, uint32_t block = step_block<Tag>()) {
 }
 
 template <class Tag, bool FORCED, bool ROWS>
-void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Tag>::raw* in,
-                   typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
+void launch_k_step_chunk(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Tag>::raw* in,
+                         typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p, float* r, float* v) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
             k_step<Tag, BC_PERIODIC, FORCED, ROWS><<<sg.grid, sg.block, sg.step_lds, L->stream>>>(in, out, p, r, v);
@@ -189,6 +191,27 @@ void launch_k_step(d3q19_lattice* L, const StepGrid& sg, const typename Codec<Ta
         default:
             k_step<Tag, BC_HALFWAY, FORCED, ROWS><<<sg.grid, sg.block, sg.step_lds, L->stream>>>(in, out, p, r, v);
             break;
+    }
+}
+
+// One fused step, launched in plane chunks on the row grid (step_chunks):
+// chunk k covers the x planes [k S, min((k + 1) S, nx)), S = chunk_planes.
+template <class Tag, bool FORCED, bool ROWS>
+void launch_k_step(d3q19_lattice* L, const StepGrid& sf, const typename Codec<Tag>::raw* in,
+                   typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p0, float* r, float* v) {
+    const uint32_t nx = (uint32_t)L->cfg.nx;
+    const uint32_t n = ROWS ? step_chunks(L->ncell, nx, step_chunk_glups<Tag>(), step_chunk_min_us<Tag>(),
+                                          STEP_CHUNK_US, L->chunk_override)
+                            : 1u;
+    const uint32_t S = chunk_planes(nx, n);
+    for (uint32_t x0 = 0; x0 < nx; x0 += S) {
+        StepGrid sg = sf;
+        Params<typename Codec<Tag>::C> p = p0;
+        if (n > 1) {
+            sg.grid.z = std::min(S, nx - x0);
+            p.x0 = x0;
+        }
+        launch_k_step_chunk<Tag, FORCED, ROWS>(L, sg, in, out, p, r, v);
     }
 }
 
@@ -783,6 +806,10 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
     }
     L->stream = (hipStream_t)stream;  // NULL: the default stream
     L->lds_per_cu = device_lds_per_cu(cfg->device);
+    if (const char* chunks = std::getenv("D3Q19_STEP_CHUNKS")) {
+        const long c = std::strtol(chunks, nullptr, 10);
+        L->chunk_override = c > 0 && c <= 65535 ? (uint32_t)c : 0u;
+    }
     const size_t lat_bytes = L->lat_bytes;
     // partial sums: one slot per block of the population grid and of the
     // linear 256-thread grids (ncell / 256)

@@ -194,6 +194,31 @@ constexpr uint32_t step_blocks_per_cu() {
     }
 }
 
+// Plane chunks of the fused step (step_chunks in d3q19_launch.hpp): the rate
+// a storage steps at (GLUPS, to estimate a step's duration), the duration
+// from which a step is chunked, and the work per chunk (us).  Measured on
+// MI355X (GLUPS, one launch -> chunks of about 210 us): FP32 256^3 40.7 ->
+// 41.3 (2 chunks), 512^3 39.2 -> 41.1 (16); FP64 256^3 19.6 -> 20.4 (4);
+// FP16 512^3 75.8 -> 78.5 (8).  Shorter launches lose more to their ramp-up
+// and tail than they gain: FP16 256^3 80.3 -> 77.9 in 2 chunks, 384^3
+// 85.9 -> 85.3 in 2, FP32 128^3 40.0 -> 37.6 in 2; so 16-bit storage
+// chunks only steps of a millisecond or more.  The moment-coded step, bound
+// by issue rather than by HBM, runs in one launch (256^3: 47.0 -> 46.5 in 2).
+template <class Tag>
+constexpr double step_chunk_glups() {
+    if constexpr (std::is_same<Tag, TME16>::value) {
+        return 0.0;
+    } else {
+        constexpr size_t es = sizeof(typename Codec<Tag>::raw);
+        return es == 8 ? 20.0 : (es == 4 ? 41.0 : 82.0);
+    }
+}
+template <class Tag>
+constexpr double step_chunk_min_us() {
+    return sizeof(typename Codec<Tag>::raw) == 2 ? 1000.0 : 300.0;
+}
+constexpr double STEP_CHUNK_US = 210.0;
+
 // ---------------------------------------------------------------------------
 // Population layout ("row-blocked"): the lattice is a sequence of z-rows
 // r = x * ny + y; row r holds the 19 direction segments of nzp elements each
@@ -232,6 +257,7 @@ template <class C> struct Params {
     // face (c_z < 0, [1]), formed as the general path forms it
     C zterm[2][19];
     const MeQuant* me;  // moment-encoded storage: its quantization constants, else null
+    uint32_t x0;        // row grid: the x plane of blockIdx.z = 0 (a fused step's plane chunk)
 };
 
 // Fill the z-wall terms of p from its face velocities (host).
@@ -278,12 +304,13 @@ __host__ __device__ inline uint32_t padded_nz(uint32_t nz) { return (nz + 63u) /
 
 // Cell coordinates of the calling thread on the (z, y, x) grid; false for
 // padding threads.  Every wavefront lies in one row (blockDim.x is a multiple
-// of 64), so y is made an SGPR value.
+// of 64), so y is made an SGPR value.  x0: the plane of blockIdx.z = 0 (a
+// launch over a chunk of planes).
 __device__ __forceinline__ bool grid_cell(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t& x,
-                                          uint32_t& y, uint32_t& z) {
+                                          uint32_t& y, uint32_t& z, uint32_t x0 = 0) {
     z = blockIdx.x * blockDim.x + threadIdx.x;
     y = __builtin_amdgcn_readfirstlane(blockIdx.y * blockDim.y + threadIdx.y);
-    x = blockIdx.z;
+    x = blockIdx.z + x0;
     return z < nz && y < ny && x < nx;
 }
 
@@ -295,9 +322,9 @@ __device__ __forceinline__ bool grid_cell(uint32_t nx, uint32_t ny, uint32_t nz,
 // row_bytes = nz * es and dir_bytes = ncell * es in the offset formula above.
 template <bool ROWS>
 __device__ __forceinline__ bool cell_of(uint32_t nx, uint32_t ny, uint32_t nz, uint32_t ncell, uint32_t& x,
-                                        uint32_t& y, uint32_t& z) {
+                                        uint32_t& y, uint32_t& z, uint32_t x0 = 0) {
     if constexpr (ROWS) {
-        return grid_cell(nx, ny, nz, x, y, z);
+        return grid_cell(nx, ny, nz, x, y, z, x0);
     } else {
         const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
         if (c >= ncell) return false;

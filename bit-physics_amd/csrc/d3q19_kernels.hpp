@@ -30,7 +30,7 @@ static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
     uint32_t x, y, z;
-    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z)) return;
+    if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z, p.x0)) return;
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     // Wavefront-uniform segment bases (seg_base) for 16-bit storage on the
     // row grid; for FP32 the same store form made the periodic step slower

@@ -16,4 +16,27 @@ inline uint32_t reserve_lds(uint32_t lds_per_cu, uint32_t k) {
     return (b <= lds_per_cu / k && b <= 65536) ? b : 0;
 }
 
+// Launches one fused step is split into along x (plane chunks, each a run of
+// whole x planes).  Within one long launch of the step the wavefronts slow
+// down as it proceeds (256^3 FP32, per-wavefront timestamps: 3.77 us in its
+// first tenth, 3.97 us in its last); a new launch starts fast again.  The
+// step is therefore launched in chunks of about target_us of work, from the
+// rate est_glups the storage steps at, once the whole step would take
+// min_us or more; at most one chunk per plane.  override_n > 0 forces that
+// many chunks (clamped to nx).
+inline uint32_t step_chunks(uint64_t ncell, uint32_t nx, double est_glups, double min_us, double target_us,
+                            uint32_t override_n = 0) {
+    if (nx == 0) return 1;
+    if (override_n > 0) return override_n < nx ? override_n : nx;
+    if (est_glups <= 0.0 || target_us <= 0.0) return 1;
+    const double us = (double)ncell / (est_glups * 1e3);
+    if (us < min_us) return 1;
+    uint32_t n = (uint32_t)(us / target_us + 0.5);
+    if (n < 1) n = 1;
+    return n < nx ? n : nx;
+}
+
+// Planes per chunk for n chunks of nx planes (the last chunk takes the rest).
+inline uint32_t chunk_planes(uint32_t nx, uint32_t n) { return n <= 1 ? nx : (nx + n - 1) / n; }
+
 }  // namespace d3q19

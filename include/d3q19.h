@@ -100,7 +100,11 @@ void d3q19_config_default(d3q19_config* cfg);
  *   f = bgk_step(f, tau, force); f = apply_bounce_back_y_walls(f, top, bottom)
  * with one fused stream-collide kernel per step.  A lattice holds at most
  * 2^32 - 1 cells (nx, ny <= 65535); host arrays passed to it are staged
- * through device buffers of their size, device arrays are used in place. */
+ * through device buffers of their size, device arrays are used in place.
+ * A large lattice's step is launched as several kernels over runs of x
+ * planes (one wavefront's results do not depend on it); the environment
+ * variable D3Q19_STEP_CHUNKS, read here, forces the number of such launches
+ * per step for the lattice (unset or 0: chosen from size and storage). */
 int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out);
 int d3q19_destroy(d3q19_lattice* lat);
 /* Launch the lattice's later work on `stream` (NULL: the default stream).  Work

@@ -1,6 +1,7 @@
 // Host-only checks of the launch-shape arithmetic (d3q19_launch.hpp): a
 // reservation of reserve_lds(L, k) bytes per block leaves room for exactly k
-// blocks on a CU with L bytes of LDS, and stays within 64 KiB per block.
+// blocks on a CU with L bytes of LDS, and stays within 64 KiB per block; the
+// plane chunks of a fused step.
 #include <cstdio>
 #include <initializer_list>
 
@@ -29,6 +30,48 @@ int main() {
     if (d3q19::reserve_lds(163840u, 1) != 0 || d3q19::reserve_lds(0, 3) != 0 || d3q19::reserve_lds(163840u, 0) != 0) {
         std::printf("reservation where none is possible or asked for\n");
         ++bad;
+    }
+    // plane chunks of a fused step: about target_us of work each once the
+    // step takes min_us or more, never more chunks than planes; the chunks
+    // of chunk_planes planes cover the x extent exactly once
+    struct Case { unsigned long long ncell; unsigned nx; double glups, min_us; unsigned want; };
+    const Case cases[] = {
+        {256ull * 256 * 256, 256, 41.0, 300.0, 2},    // FP32 256^3: 409 us
+        {256ull * 256 * 256, 256, 20.0, 300.0, 4},    // FP64 256^3: 839 us
+        {512ull * 512 * 512, 512, 41.0, 300.0, 16},   // FP32 512^3
+        {384ull * 384 * 384, 384, 41.0, 300.0, 7},    // FP32 384^3
+        {128ull * 128 * 128, 128, 41.0, 300.0, 1},    // FP32 128^3: 51 us
+        {256ull * 256 * 256, 256, 82.0, 1000.0, 1},   // FP16 256^3: 205 us
+        {384ull * 384 * 384, 384, 82.0, 1000.0, 1},   // FP16 384^3: 690 us
+        {512ull * 512 * 512, 512, 82.0, 1000.0, 8},   // FP16 512^3
+        {4000000000ull, 3, 20.0, 300.0, 3},           // more work than planes: one chunk per plane
+        {256ull * 256 * 256, 256, 0.0, 300.0, 1},     // no rate: one launch
+    };
+    for (const Case& c : cases) {
+        const unsigned n = d3q19::step_chunks(c.ncell, c.nx, c.glups, c.min_us, 210.0);
+        if (n != c.want) {
+            std::printf("step_chunks(%llu cells, nx %u, %.0f GLUPS) = %u, want %u\n", c.ncell, c.nx, c.glups, n, c.want);
+            ++bad;
+        }
+    }
+    if (d3q19::step_chunks(256ull * 256 * 256, 256, 41.0, 300.0, 210.0, 5) != 5 ||
+        d3q19::step_chunks(1000, 10, 41.0, 300.0, 210.0, 64) != 10) {
+        std::printf("step_chunks ignores or does not clamp the override\n");
+        ++bad;
+    }
+    for (unsigned nx = 1; nx <= 600; ++nx) {
+        for (unsigned n = 1; n <= nx; ++n) {
+            const unsigned S = d3q19::chunk_planes(nx, n);
+            unsigned covered = 0, launches = 0;
+            for (unsigned x0 = 0; x0 < nx; x0 += S) {
+                covered += (S < nx - x0 ? S : nx - x0);
+                ++launches;
+            }
+            if (covered != nx || launches > n || S == 0) {
+                std::printf("chunk_planes(%u, %u) = %u covers %u planes in %u launches\n", nx, n, S, covered, launches);
+                ++bad;
+            }
+        }
     }
     std::printf(bad ? "FAIL\n" : "OK\n");
     return bad ? 1 : 0;

@@ -1,7 +1,8 @@
 """Host-only test of the launch-shape arithmetic in
 bit-physics_amd/csrc/d3q19_launch.hpp (tests/cpp/test_launch.cpp, built with
 the host C++ compiler): the LDS reservation that caps the fused step's
-resident blocks per CU leaves room for exactly the intended number."""
+resident blocks per CU leaves room for exactly the intended number, and the
+plane chunks a large step is launched in cover the lattice once."""
 
 from __future__ import annotations
 
