@@ -194,15 +194,21 @@ void launch_k_step_chunk(d3q19_lattice* L, const StepGrid& sg, const typename Co
     }
 }
 
+// Plane chunks of the lattice's fused step (1 on the linear grid of short rows).
+template <class Tag>
+uint32_t fused_step_chunks(const d3q19_lattice* L) {
+    return L->rows ? step_chunks(L->ncell, (uint32_t)L->cfg.nx, step_chunk_glups<Tag>(), step_chunk_min_us<Tag>(),
+                                 STEP_CHUNK_US, L->chunk_override)
+                   : 1u;
+}
+
 // One fused step, launched in plane chunks on the row grid (step_chunks):
 // chunk k covers the x planes [k S, min((k + 1) S, nx)), S = chunk_planes.
 template <class Tag, bool FORCED, bool ROWS>
 void launch_k_step(d3q19_lattice* L, const StepGrid& sf, const typename Codec<Tag>::raw* in,
                    typename Codec<Tag>::raw* out, const Params<typename Codec<Tag>::C>& p0, float* r, float* v) {
     const uint32_t nx = (uint32_t)L->cfg.nx;
-    const uint32_t n = ROWS ? step_chunks(L->ncell, nx, step_chunk_glups<Tag>(), step_chunk_min_us<Tag>(),
-                                          STEP_CHUNK_US, L->chunk_override)
-                            : 1u;
+    const uint32_t n = ROWS ? fused_step_chunks<Tag>(L) : 1u;
     const uint32_t S = chunk_planes(nx, n);
     for (uint32_t x0 = 0; x0 < nx; x0 += S) {
         StepGrid sg = sf;
@@ -1261,6 +1267,20 @@ int d3q19_synchronize(d3q19_lattice* L) {
 }
 
 int64_t d3q19_step_count(const d3q19_lattice* L) { return L ? L->steps : -1; }
+
+int32_t d3q19_step_launches(const d3q19_lattice* L) {
+    if (!L) return -1;
+    if (L->me_ref) return 2;
+    switch (L->cfg.storage) {
+        case D3Q19_F64: return (int32_t)fused_step_chunks<TF64>(L);
+        case D3Q19_F32: return (int32_t)fused_step_chunks<TF32>(L);
+        case D3Q19_F16: return (int32_t)fused_step_chunks<TF16>(L);
+        case D3Q19_BF16: return (int32_t)fused_step_chunks<TBF16>(L);
+        case D3Q19_BF16SR: return (int32_t)fused_step_chunks<TBF16S>(L);
+        case D3Q19_ME16: return (int32_t)fused_step_chunks<TME16>(L);
+    }
+    return -1;
+}
 size_t d3q19_device_bytes(const d3q19_lattice* L) { return L ? L->bytes : 0; }
 
 // ---------------------------------------------------------------------------

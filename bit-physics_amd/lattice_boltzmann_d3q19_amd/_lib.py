@@ -38,7 +38,7 @@ EXPORTED = (
     "d3q19_adjoint_checkpoint_every", "d3q19_adjoint_memory_bytes", "d3q19_copy_kernel",
     "d3q19_moment_basis", "d3q19_set_moment_ranges", "d3q19_moment_range", "d3q19_moment_range_stored", "d3q19_health",
     "d3q19_set_step_count", "d3q19_state_bytes", "d3q19_read_state", "d3q19_write_state",
-    "d3q19_feq", "d3q19_shader_clock", "d3q19_read_moments_noalloc",
+    "d3q19_feq", "d3q19_shader_clock", "d3q19_read_moments_noalloc", "d3q19_step_launches",
 )
 
 
@@ -125,6 +125,7 @@ def load() -> ctypes.CDLL:
         "d3q19_enstrophy": (ctypes.c_int, [P, P]),
         "d3q19_synchronize": (ctypes.c_int, [P]),
         "d3q19_step_count": (i64, [P]),
+        "d3q19_step_launches": (i32, [P]),
         "d3q19_device_bytes": (ctypes.c_size_t, [P]),
         "d3q19_bgk_step": (ctypes.c_int, [P, P, P, i32, i32, i32, f64, i32, P]),
         "d3q19_stream": (ctypes.c_int, [P, P, i32, i32, i32, i32, P]),

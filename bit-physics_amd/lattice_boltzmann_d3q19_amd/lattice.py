@@ -270,6 +270,11 @@ class Lattice:
         return int(load().d3q19_step_count(self._h))
 
     @property
+    def step_launches(self) -> int:
+        """Kernel launches one step makes (the fused step's plane chunks)."""
+        return int(load().d3q19_step_launches(self._h))
+
+    @property
     def device_bytes(self) -> int:
         return int(load().d3q19_device_bytes(self._h))
 

@@ -163,6 +163,11 @@ int d3q19_health(d3q19_lattice* lat, double out[4]);
 int d3q19_enstrophy(d3q19_lattice* lat, double* out);
 int d3q19_synchronize(d3q19_lattice* lat);
 int64_t d3q19_step_count(const d3q19_lattice* lat);
+/* Kernel launches one step makes once the state is streamed: the fused
+ * step's plane chunks (see d3q19_create), 2 for D3Q19_ME16_REF (decode +
+ * collide, stream + encode); -1 for a null lattice.  (No reference
+ * counterpart: it tells a profile's per-launch figures from per-step ones.) */
+int32_t d3q19_step_launches(const d3q19_lattice* lat);
 /* Set the step counter (the BF16SR rounding bits are keyed on it). */
 int d3q19_set_step_count(d3q19_lattice* lat, int64_t n);
 
