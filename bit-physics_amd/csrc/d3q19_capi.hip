@@ -100,6 +100,18 @@ struct d3q19_lattice {
     double* mr_partials = nullptr;  // moment-range partials and result
     double* mr_out = nullptr;
     uint32_t chunk_override = 0;  // plane chunks per fused step, 0 = by step_chunks (D3Q19_STEP_CHUNKS)
+    // D3Q19_ME16 on the row grid: the codes live in one ring of ring_np x
+    // planes (buf[0]; buf[1] unused), the state's plane 0 at ring position
+    // ring_base; a step writes the next state ring_gap planes before it, in
+    // place (me_row_in, me_row_out, me_ring_sync).
+    bool ring = false;
+    uint32_t ring_np = 0, ring_base = 0, ring_gap = 0;
+    uint32_t ring_blocks = 0;       // blocks of the fused step per plane
+    uint32_t ring_epoch = 0;        // ring steps taken (ring_done reaches ring_blocks per step)
+    size_t plane_bytes = 0;         // one x plane of the row-blocked layout
+    void* ring_saved = nullptr;     // plane 0 of the state a step reads
+    uint32_t* ring_done = nullptr;  // nx counters, ME_RING_STRIDE apart
+    uint32_t* ring_err = nullptr;   // set by a wait that gave up
     int64_t steps = 0;
     size_t bytes = 0;
     uint32_t lds_per_cu = 0;  // LDS bytes per CU of the device, 0 if unknown
@@ -134,12 +146,16 @@ Params<C> make_params(const d3q19_lattice* L) {
     set_zwall_terms(p);
     p.me = L->mq;
     p.seed = (uint32_t)L->steps * 0x9e3779b9u + 0x7f4a7c15u;
+    if (L->ring) {
+        p.ring_np = L->ring_np;
+        p.ring_in = p.ring_out = L->ring_base;
+    }
     return p;
 }
 
 template <class Tag>
 typename Codec<Tag>::raw* buf_of(const d3q19_lattice* L, int which) {
-    return (typename Codec<Tag>::raw*)L->buf[which];
+    return (typename Codec<Tag>::raw*)L->buf[L->ring ? 0 : which];
 }
 
 // Launch shape of every population kernel: (z, y, x) blocks whose x extent
@@ -234,16 +250,34 @@ int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
         float* r = (s == n - 1) ? rho : nullptr;
         float* v = (s == n - 1) ? u : nullptr;
         if (L->local) {
-            // first step after a load: collide f^n in place, no streaming
+            // first step after a load: collide f^n in place, no streaming (a
+            // ring collides each cell into its own slots: in == out)
             k_collide_local<Tag, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, out, p, r, v);
             L->local = false;
-        } else if (forced) {
-            launch_k_step<Tag, true, ROWS>(L, sf, in, out, p, r, v);
         } else {
-            launch_k_step<Tag, false, ROWS>(L, sf, in, out, p, r, v);
+            Params<C> q = p;
+            if (L->ring) {
+                // plane 0 of the state read, for the last planes' pulls once its slot is reused
+                HIPCHK(hipMemcpyAsync(L->ring_saved, (const char*)L->buf[0] + (size_t)L->ring_base * L->plane_bytes,
+                                      L->plane_bytes, hipMemcpyDeviceToDevice, L->stream));
+                q.ring_out = (L->ring_base + L->ring_np - L->ring_gap) % L->ring_np;
+                q.ring_gap = L->ring_gap;
+                q.ring_target = L->ring_blocks * (L->ring_epoch + 1u);
+                q.ring_saved = (const char*)L->ring_saved;
+                q.ring_done = L->ring_done;
+                q.ring_err = L->ring_err;
+            }
+            if (forced)
+                launch_k_step<Tag, true, ROWS>(L, sf, in, out, q, r, v);
+            else
+                launch_k_step<Tag, false, ROWS>(L, sf, in, out, q, r, v);
+            if (L->ring) {
+                L->ring_base = q.ring_out;
+                L->ring_epoch++;
+            }
         }
         HIPCHK(hipGetLastError());
-        L->cur ^= 1;
+        if (!L->ring) L->cur ^= 1;
         L->steps++;
     }
     return D3Q19_OK;
@@ -753,6 +787,37 @@ int scratch_on(int dev, void** out) {
     return D3Q19_OK;
 }
 
+// Copy the state of a moment-coded ring between host memory (planes 0 .. nx - 1
+// in order, the layout of a two-lattice state) and the ring: at most two
+// contiguous runs of planes.
+int ring_copy(d3q19_lattice* L, void* host, hipMemcpyKind kind) {
+    const uint32_t nx = (uint32_t)L->cfg.nx;
+    const uint32_t first = std::min(nx, L->ring_np - L->ring_base);
+    char* ring = (char*)L->buf[0];
+    char* h = (char*)host;
+    const size_t pb = L->plane_bytes;
+    if (kind == hipMemcpyDeviceToHost) {
+        HIPCHK(hipMemcpyAsync(h, ring + (size_t)L->ring_base * pb, (size_t)first * pb, kind, L->stream));
+        if (first < nx) HIPCHK(hipMemcpyAsync(h + (size_t)first * pb, ring, (size_t)(nx - first) * pb, kind, L->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync(ring + (size_t)L->ring_base * pb, h, (size_t)first * pb, kind, L->stream));
+        if (first < nx) HIPCHK(hipMemcpyAsync(ring, h + (size_t)first * pb, (size_t)(nx - first) * pb, kind, L->stream));
+    }
+    return D3Q19_OK;
+}
+
+// After a synchronisation: whether a ring step's wait gave up (its result
+// would be unreliable).  The flag stays set.
+int ring_check(d3q19_lattice* L) {
+    if (!L->ring) return D3Q19_OK;
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, L->ring_err, sizeof(err), hipMemcpyDeviceToHost));
+    if (err)
+        return fail(D3Q19_EHIP, "a moment-coded step waited too long for the pulls of the planes it overwrites; "
+                                "the lattice state is unreliable");
+    return D3Q19_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -817,23 +882,52 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
         L->chunk_override = c > 0 && c <= 65535 ? (uint32_t)c : 0u;
     }
     const size_t lat_bytes = L->lat_bytes;
+    // Moment codes on the row grid: one ring of nx + gap planes instead of
+    // two lattices (the gap from the blocks a CU holds: at most 4 of the
+    // moment-coded step's 512-thread blocks).
+    if (me && L->rows) {
+        const StepGrid sf = step_grid<TME16>(L, fused_block<TME16>());
+        const uint64_t resident = (uint64_t)std::max(device_cu_count(cfg->device), 1u) * 4u;
+        L->ring = true;
+        L->ring_gap = ring_gap((uint32_t)cfg->nx, (uint64_t)sf.grid.x * sf.grid.y, resident);
+        if (const char* gap = std::getenv("D3Q19_RING_GAP")) {  // a test knob: a tighter gap
+            const long g = std::strtol(gap, nullptr, 10);
+            if (g >= 2 && g < (long)L->ring_gap) L->ring_gap = (uint32_t)g;
+        }
+        L->ring_np = (uint32_t)cfg->nx + L->ring_gap;
+        L->plane_bytes = (size_t)cfg->ny * L->row_bytes;
+        L->ring_blocks = sf.grid.x * sf.grid.y;
+    }
+    const size_t buf0_bytes = L->ring ? (size_t)L->ring_np * L->plane_bytes : lat_bytes;
     // partial sums: one slot per block of the population grid and of the
     // linear 256-thread grids (ncell / 256)
     const uint32_t nb = (uint32_t)std::max<uint64_t>(
         nblocks(ncell), L->rows ? (uint64_t)cfg->nx * cfg->ny * ((cfg->nz + 63) / 64) : 0);
-    if (hipMalloc(&L->buf[0], lat_bytes) != hipSuccess || hipMalloc(&L->buf[1], lat_bytes) != hipSuccess ||
+    if (hipMalloc(&L->buf[0], buf0_bytes) != hipSuccess ||
+        (!L->ring && hipMalloc(&L->buf[1], lat_bytes) != hipSuccess) ||
+        (L->ring && (hipMalloc(&L->ring_saved, L->plane_bytes) != hipSuccess ||
+                     hipMalloc((void**)&L->ring_done, (size_t)cfg->nx * ME_RING_STRIDE * sizeof(uint32_t)) != hipSuccess ||
+                     hipMalloc((void**)&L->ring_err, sizeof(uint32_t)) != hipSuccess)) ||
         hipMalloc((void**)&L->partials, (size_t)nb * NDIAG * sizeof(double)) != hipSuccess ||
         hipMalloc((void**)&L->dout, 8 * sizeof(double)) != hipSuccess ||
         (me && hipMalloc((void**)&L->mq, sizeof(MeQuant)) != hipSuccess)) {
         d3q19_destroy(L);
         return fail(D3Q19_ENOMEM, "device allocation of the lattice failed");
     }
-    L->bytes = 2 * lat_bytes + (size_t)nb * NDIAG * sizeof(double) + 64 + (me ? sizeof(MeQuant) : 0);
+    if (L->ring && (hipMemset(L->ring_done, 0, (size_t)cfg->nx * ME_RING_STRIDE * sizeof(uint32_t)) != hipSuccess ||
+                    hipMemset(L->ring_err, 0, sizeof(uint32_t)) != hipSuccess)) {
+        d3q19_destroy(L);
+        return fail(D3Q19_EHIP, "clearing the ring counters failed");
+    }
+    L->bytes = (L->ring ? buf0_bytes + L->plane_bytes + ((size_t)cfg->nx * ME_RING_STRIDE + 1) * sizeof(uint32_t)
+                        : 2 * lat_bytes) +
+               (size_t)nb * NDIAG * sizeof(double) + 64 + (me ? sizeof(MeQuant) : 0);
     // Row padding (z >= nz of a padded row) is never written by a kernel:
     // zero it once, so that the raw state (read_state, checkpoints) is a
     // function of the lattice's content alone.
     if (L->rows && L->nzp != (uint32_t)cfg->nz) {
-        if (hipMemset(L->buf[0], 0, lat_bytes) != hipSuccess || hipMemset(L->buf[1], 0, lat_bytes) != hipSuccess) {
+        if (hipMemset(L->buf[0], 0, buf0_bytes) != hipSuccess ||
+            (L->buf[1] && hipMemset(L->buf[1], 0, lat_bytes) != hipSuccess)) {
             d3q19_destroy(L);
             return fail(D3Q19_EHIP, "clearing the lattice padding failed");
         }
@@ -845,8 +939,9 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
 int d3q19_destroy(d3q19_lattice* L) {
     if (!L) return D3Q19_OK;
     (void)hipStreamSynchronize(L->stream);
-    void* ps[] = {L->buf[0], L->buf[1],   L->force,       L->brink,  L->solid, L->partials,
-                  L->dout,   L->uscratch, L->mr_partials, L->mr_out, L->ref_b, L->mq};
+    void* ps[] = {L->buf[0],      L->buf[1],    L->force,       L->brink,     L->solid,
+                  L->partials,    L->dout,      L->uscratch,    L->mr_partials, L->mr_out,
+                  L->ref_b,       L->mq,        L->ring_saved,  L->ring_done, L->ring_err};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     delete L;
@@ -1025,7 +1120,7 @@ int d3q19_read_populations(d3q19_lattice* L, void* f, int32_t dt, int32_t on_dev
     if (rc) return rc;
     if (!on_device) HIPCHK(hipMemcpyAsync(f, dst, n * es, hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_read_moments(d3q19_lattice* L, void* rho, void* u, int32_t dt, int32_t on_device) {
@@ -1058,7 +1153,7 @@ int d3q19_read_moments(d3q19_lattice* L, void* rho, void* u, int32_t dt, int32_t
         if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * nc * es, hipMemcpyDeviceToHost, L->stream));
     }
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_read_moments_noalloc(d3q19_lattice* L, void* rho, void* u, int32_t dt) {
@@ -1088,7 +1183,7 @@ int d3q19_read_moments_noalloc(d3q19_lattice* L, void* rho, void* u, int32_t dt)
     if (rho) HIPCHK(hipMemcpyAsync(rho, drho, nc * es, hipMemcpyDeviceToHost, L->stream));
     if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * nc * es, hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_diagnostics(d3q19_lattice* L, double out[5]) {
@@ -1100,7 +1195,7 @@ int d3q19_diagnostics(d3q19_lattice* L, double out[5]) {
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(out, L->dout, NDIAG * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_enstrophy(d3q19_lattice* L, double* out) {
@@ -1136,7 +1231,7 @@ int d3q19_enstrophy(d3q19_lattice* L, double* out) {
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, L->dout, sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_moment_basis(int32_t m[361], int32_t minv4[361]) {
@@ -1219,9 +1314,15 @@ int d3q19_read_state(d3q19_lattice* L, void* dst, int32_t* local, int64_t* steps
     if (int e = check_lat(L)) return e;
     if (!dst) return fail(D3Q19_EINVAL, "null output");
     if (!L->has_state) return fail(D3Q19_ESTATE, "lattice has no state");
-    const void* src = L->me_ref ? L->buf[0] : L->buf[L->cur];
-    HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyDeviceToHost, L->stream));
+    if (L->ring) {
+        // planes 0 .. nx - 1 in order: the ring positions base .. base + nx - 1, wrapped
+        if (int e = ring_copy(L, dst, hipMemcpyDeviceToHost)) return e;
+    } else {
+        const void* src = L->me_ref ? L->buf[0] : L->buf[L->cur];
+        HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyDeviceToHost, L->stream));
+    }
     HIPCHK(hipStreamSynchronize(L->stream));
+    if (int e = ring_check(L)) return e;
     if (local) *local = L->local ? 1 : 0;
     if (steps) *steps = L->steps;
     return D3Q19_OK;
@@ -1233,8 +1334,12 @@ int d3q19_write_state(d3q19_lattice* L, const void* src, int32_t local, int64_t 
     if (steps < 0) return fail(D3Q19_EINVAL, "negative step count");
     if ((L->me || L->me_ref) && !L->me_ranges)
         return fail(D3Q19_ESTATE, "a moment-encoded lattice needs d3q19_set_moment_ranges first");
-    void* dst = L->me_ref ? L->buf[0] : L->buf[L->cur];
-    HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyHostToDevice, L->stream));
+    if (L->ring) {
+        if (int e = ring_copy(L, const_cast<void*>(src), hipMemcpyHostToDevice)) return e;
+    } else {
+        void* dst = L->me_ref ? L->buf[0] : L->buf[L->cur];
+        HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyHostToDevice, L->stream));
+    }
     HIPCHK(hipStreamSynchronize(L->stream));
     L->has_state = true;
     L->local = L->me_ref ? true : local != 0;
@@ -1257,13 +1362,13 @@ int d3q19_health(d3q19_lattice* L, double out[4]) {
     if (int rc = D3Q19_DISPATCH(L, launch_health, L)) return rc;
     HIPCHK(hipMemcpyAsync(out, L->dout, NHEALTH * sizeof(double), hipMemcpyDeviceToHost, L->stream));
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int d3q19_synchronize(d3q19_lattice* L) {
     if (int e = check_lat(L)) return e;
     HIPCHK(hipStreamSynchronize(L->stream));
-    return D3Q19_OK;
+    return ring_check(L);
 }
 
 int64_t d3q19_step_count(const d3q19_lattice* L) { return L ? L->steps : -1; }

@@ -258,6 +258,15 @@ template <class C> struct Params {
     C zterm[2][19];
     const MeQuant* me;  // moment-encoded storage: its quantization constants, else null
     uint32_t x0;        // row grid: the x plane of blockIdx.z = 0 (a fused step's plane chunk)
+    // Moment codes in one ring of planes (me_row_in / me_row_out, me_ring_sync):
+    uint32_t ring_np;        // planes in the ring, 0: the plain layout
+    uint32_t ring_in;        // ring position of plane 0 of the state read
+    uint32_t ring_out;       // ring position of plane 0 of the state written
+    uint32_t ring_gap;       // step: the state written lies ring_gap planes before the one read
+    uint32_t ring_target;    // step: the count ring_done[x] reaches once plane x is read
+    const char* ring_saved;  // step: plane 0 of the state read, copied before the step
+    uint32_t* ring_done;     // step: per plane, wavefronts whose pulls are consumed (monotonic)
+    uint32_t* ring_err;      // step: set to 1 when a wait gave up
 };
 
 // Fill the z-wall terms of p from its face velocities (host).
@@ -343,6 +352,34 @@ __device__ __forceinline__ uint32_t block_linear_id() {
 template <class C>
 __device__ __forceinline__ const char* row_of(const void* base, const Params<C>& p, uint32_t x, uint32_t y) {
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;
+}
+
+// Moment codes in a ring of ring_np planes (one buffer).  A step reads the
+// state at ring position ring_in and writes the next one ring_gap planes
+// before it, in place: plane x of the new state overwrites plane x - gap of
+// the old one once the wavefronts that pull from that plane have consumed
+// their codes (me_ring_sync), and plane 0 of the old state, which the last
+// planes pull from after its slot is reused, is read from a copy made
+// before the step (ring_saved).  Outside a step ring_in = ring_out.
+// Plane x of the state read (its row y at + y * row_bytes).
+template <class C>
+__device__ __forceinline__ const char* me_plane_in(const void* base, const Params<C>& p, uint32_t x) {
+    if (p.ring_np == 0) return (const char*)base + (uint64_t)(x * p.ny) * p.row_bytes;
+    if (p.ring_saved && x == 0) return p.ring_saved;
+    uint32_t pos = x + p.ring_in;
+    if (pos >= p.ring_np) pos -= p.ring_np;
+    return (const char*)base + (uint64_t)(pos * p.ny) * p.row_bytes;
+}
+template <class C>
+__device__ __forceinline__ const char* me_row_in(const void* base, const Params<C>& p, uint32_t x, uint32_t y) {
+    return me_plane_in(base, p, x) + (uint64_t)y * p.row_bytes;
+}
+template <class C>
+__device__ __forceinline__ char* me_row_out(void* base, const Params<C>& p, uint32_t x, uint32_t y) {
+    if (p.ring_np == 0) return (char*)row_of(base, p, x, y);
+    uint32_t pos = x + p.ring_out;
+    if (pos >= p.ring_np) pos -= p.ring_np;
+    return (char*)base + (uint64_t)(pos * p.ny + y) * p.row_bytes;
 }
 
 // U: the address of a direction segment of a row, wavefront-uniform (the row
@@ -543,11 +580,12 @@ template <class Tag, bool U = false>
 __device__ __forceinline__ void load_local(const void* in, const Params<typename Codec<Tag>::C>& p, uint32_t x,
                                            uint32_t y, uint32_t z, typename Codec<Tag>::C d[Q]) {
     constexpr uint32_t ES = sizeof(typename Codec<Tag>::raw);
-    const char* row = row_of(in, p, x, y);
     if constexpr (std::is_same<Tag, TME16>::value) {
+        const char* row = me_row_in(in, p, x, y);
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = me_pop<U>(row, i, p.dir_bytes, z * ES, p.me);
     } else {
+        const char* row = row_of(in, p, x, y);
 #pragma unroll
         for (int i = 0; i < Q; ++i) d[i] = ld<Tag, false, U>(row, i, p.dir_bytes, z * ES);
     }
@@ -589,7 +627,7 @@ __device__ __forceinline__ void store_local(void* out, const Params<typename Cod
                                             uint32_t y, uint32_t z, const typename Codec<Tag>::C d[Q]) {
     typedef typename Codec<Tag>::raw R;
     constexpr uint32_t ES = sizeof(R);
-    char* row = (char*)row_of(out, p, x, y);
+    char* row = std::is_same<Tag, TME16>::value ? me_row_out(out, p, x, y) : (char*)row_of(out, p, x, y);
     uint32_t zoff = z * ES;
     // an opaque 32-bit lane offset: with the forced collision in the same
     // kernel LLVM otherwise widened it to 64 bits and added it per store
@@ -922,7 +960,10 @@ __device__ __forceinline__ void me_pull(const void* in, const Params<float>& p, 
     if constexpr (U) asm volatile("" : "+v"(zo[0]), "+v"(zo[1]), "+v"(zo[2]));
     if constexpr (BC == BC_HALFWAY) {
         const uint32_t wm = p.wall_mask;
-        const char* self = row_of(in, p, x, y);
+        const char* pl[3] = {me_plane_in(in, p, xs[0]), me_plane_in(in, p, xs[1]), me_plane_in(in, p, xs[2])};
+        const uint64_t yo[3] = {(uint64_t)ys[0] * p.row_bytes, (uint64_t)ys[1] * p.row_bytes,
+                                (uint64_t)ys[2] * p.row_bytes};
+        const char* self = pl[1] + yo[1];
         bool solid_src[Q];
 #pragma unroll
         for (int i = 0; i < Q; ++i) solid_src[i] = false;
@@ -938,14 +979,17 @@ __device__ __forceinline__ void me_pull(const void* in, const Params<float>& p, 
             if (f >= 0 || solid_src[i])
                 v = me_pop<U>(self, OPP[i], DB, zo[1], p.me);
             else
-                v = me_pop<U>(row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]), i, DB, zo[1 - CZ[i]], p.me);
+                v = me_pop<U>(pl[1 - CX[i]] + yo[1 - CY[i]], i, DB, zo[1 - CZ[i]], p.me);
             if (f >= 0) v = v + 6.0f * wgt<float>(i) * cdot<float>(i, p.face_u[f][0], p.face_u[f][1], p.face_u[f][2]);
             d[i] = v;
         }
     } else {
+        const char* pl[3] = {me_plane_in(in, p, xs[0]), me_plane_in(in, p, xs[1]), me_plane_in(in, p, xs[2])};
+        const uint64_t yo[3] = {(uint64_t)ys[0] * p.row_bytes, (uint64_t)ys[1] * p.row_bytes,
+                                (uint64_t)ys[2] * p.row_bytes};
 #pragma unroll
         for (int i = 0; i < Q; ++i)
-            d[i] = me_pop<U>(row_of(in, p, xs[1 - CX[i]], ys[1 - CY[i]]), i, DB, zo[1 - CZ[i]], p.me);
+            d[i] = me_pop<U>(pl[1 - CX[i]] + yo[1 - CY[i]], i, DB, zo[1 - CZ[i]], p.me);
         // Every decoded value complete here: otherwise the forced periodic
         // step kept loads in flight across the force-mode branches of the
         // collision and spilled 23 VGPRs to scratch at 128.

@@ -28,6 +28,12 @@ D3Q19_HIDDEN int stage_out(const void* src, bool src_double, size_t n, void* dst
 inline uint32_t grid_for(uint64_t n, int block) { return (uint32_t)((n + block - 1) / block); }
 
 // LDS bytes per CU of `device`, 0 if the runtime does not report it.
+inline uint32_t device_cu_count(int device) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n <= 0) return 0;
+    return (uint32_t)n;
+}
+
 inline uint32_t device_lds_per_cu(int device) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess ||

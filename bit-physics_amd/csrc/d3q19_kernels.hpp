@@ -19,6 +19,85 @@ __device__ __forceinline__ void step_moments_out(float* rho_out, float* u_out, u
     }
 }
 
+// In-place step of a moment-coded ring (me_row_in / me_row_out): the
+// blocks of plane x write into the slot of old plane q = x - gap, which
+// planes q - 1, q and q + 1 pull from; they store once every block of those
+// planes has consumed its pulls.  ring_done[x * ME_RING_STRIDE] counts the
+// blocks of plane x whose wavefronts have all consumed their codes
+// (monotonic over steps: ring_target after this step), one counter per
+// 256-byte line; within a block an LDS count finds the last live wavefront,
+// which adds the block.  The first gap planes write into free slots, and
+// the slot of old plane 0 is free as well (its pulls read the copy
+// ring_saved).  The block's thread (0, 0) loads the three counters when the
+// block starts and checks them when its own cells are collided (they are
+// long done: the gap spans more planes than the blocks resident at once),
+// then raises an LDS flag that the other wavefronts wait on; a short count
+// makes it spin, and give up (ring_err) after about 10 ms.  Waits only
+// ever point at lower planes, which are dispatched earlier, and a block adds
+// itself before it waits.  Measured at 256^3 (GLUPS): without any wait
+// 44.8 (two lattices: 46-47); counted per wavefront on packed counters 2.3,
+// per block on packed counters with every wavefront reading them 8.8 -- the
+// operations on one counter line serialise; a counter per line with one
+// wavefront per block reading them when it got there, 41.8.
+constexpr uint32_t ME_RING_MAX_SPINS = 40000;
+constexpr uint32_t ME_RING_STRIDE = 64;  // uint32 per counter: one 256-byte line each
+
+__device__ __forceinline__ bool me_ring_short(uint32_t count, uint32_t target) {
+    return (int32_t)(count - target) < 0;
+}
+
+// The counts of the three planes a block of plane x waits for, loaded by the
+// block's thread (0, 0) -- a cell of every block -- when the block starts,
+// so that their latency hides behind the pull (0 where there is no wait).
+struct MeRingDeps {
+    uint32_t c[3] = {0u, 0u, 0u};
+};
+__device__ __forceinline__ MeRingDeps me_ring_deps(const Params<float>& p, uint32_t x) {
+    MeRingDeps d;
+    if (p.ring_done && x > p.ring_gap && threadIdx.x == 0 && threadIdx.y == 0) {
+#pragma unroll
+        for (uint32_t k = 0; k < 3; ++k)
+            d.c[k] = __hip_atomic_load(p.ring_done + (size_t)(x - p.ring_gap - 1 + k) * ME_RING_STRIDE,
+                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return d;
+}
+
+// lds[0]: wavefronts of the block done with their pulls; lds[1]: dependencies checked.
+__device__ __forceinline__ void me_ring_sync(const Params<float>& p, uint32_t x, uint32_t live_waves, uint32_t* lds,
+                                             const MeRingDeps& dep) {
+    asm volatile("" ::: "memory");  // the pulls above are consumed by the collision
+    if (__lane_id() == 0) {
+        const uint32_t n = __hip_atomic_fetch_add(lds, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (n + 1 == live_waves)
+            __hip_atomic_fetch_add(p.ring_done + (size_t)x * ME_RING_STRIDE, 1u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        if (x > p.ring_gap) {
+            if (threadIdx.x == 0 && threadIdx.y == 0) {
+                for (uint32_t k = 0; k < 3; ++k) {
+                    const uint32_t* c = p.ring_done + (size_t)(x - p.ring_gap - 1 + k) * ME_RING_STRIDE;
+                    uint32_t seen = dep.c[k], spins = 0;
+                    while (me_ring_short(seen, p.ring_target)) {
+                        if (++spins > ME_RING_MAX_SPINS) {
+                            __hip_atomic_store(p.ring_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(4);
+                        seen = __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                }
+                __hip_atomic_store(lds + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            } else {
+                uint32_t spins = 0;
+                while (__hip_atomic_load(lds + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0 &&
+                       ++spins < 4 * ME_RING_MAX_SPINS)
+                    __builtin_amdgcn_s_sleep(2);
+            }
+        }
+    }
+    asm volatile("" ::: "memory");  // no store of this wavefront above the wait
+}
+
 // ---------------------------------------------------------------------------
 // The hot path: one fused pull-stream / boundary / moments / BGK+Guo pass.
 // Reads the post-collision state of step n-1, writes that of step n, and
@@ -29,8 +108,18 @@ static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag
     k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
     typedef typename Codec<Tag>::C C;
+    constexpr bool RING = std::is_same<Tag, TME16>::value && ROWS;  // a moment-coded ring may step in place
+    __shared__ uint32_t ring_lds[RING ? 2 : 1];
+    if constexpr (RING) {
+        if (p.ring_done) {  // uniform: every thread of the block, before any returns
+            if (threadIdx.x == 0 && threadIdx.y == 0) ring_lds[0] = ring_lds[1] = 0u;
+            __syncthreads();
+        }
+    }
     uint32_t x, y, z;
     if (!cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z, p.x0)) return;
+    MeRingDeps ring_deps;
+    if constexpr (RING) ring_deps = me_ring_deps(p, x);
     const uint32_t cell = (x * p.ny + y) * p.nz + z;
     // Wavefront-uniform segment bases (seg_base) for 16-bit storage on the
     // row grid; for FP32 the same store form made the periodic step slower
@@ -63,12 +152,28 @@ static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag
             return;
         }
     }
-    if (BC == BC_HALFWAY && p.solid && p.solid[cell]) {
+    if constexpr (RING) {
+        if (BC == BC_HALFWAY && p.solid && p.solid[cell]) {
 #pragma unroll
-        for (int i = 0; i < Q; ++i) d[i] = C(0);
+            for (int i = 0; i < Q; ++i) d[i] = C(0);
+        } else {
+            pull<Tag, BC, false, U>(in, p, x, y, z, d);
+            collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+        }
+        if (p.ring_done) {
+            // live wavefronts of the block: its rows below ny times the wavefronts of its z segment below nz
+            const uint32_t rows = min(blockDim.y, p.ny - blockIdx.y * blockDim.y);
+            const uint32_t zs = min(blockDim.x, p.nz - blockIdx.x * blockDim.x);
+            me_ring_sync(p, x, rows * ((zs + 63u) / 64u), ring_lds, ring_deps);
+        }
     } else {
-        pull<Tag, BC, false, U>(in, p, x, y, z, d);
-        collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+        if (BC == BC_HALFWAY && p.solid && p.solid[cell]) {
+#pragma unroll
+            for (int i = 0; i < Q; ++i) d[i] = C(0);
+        } else {
+            pull<Tag, BC, false, U>(in, p, x, y, z, d);
+            collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
+        }
     }
     store_local<Tag, U>(out, p, x, y, z, d);
     step_moments_out(rho_out, u_out, p.ncell, cell, rho, ux, uy, uz);

@@ -39,4 +39,16 @@ inline uint32_t step_chunks(uint64_t ncell, uint32_t nx, double est_glups, doubl
 // Planes per chunk for n chunks of nx planes (the last chunk takes the rest).
 inline uint32_t chunk_planes(uint32_t nx, uint32_t n) { return n <= 1 ? nx : (nx + n - 1) / n; }
 
+// Gap, in x planes, between the state a moment-coded ring step reads and the
+// one it writes in place (me_row_in / me_row_out): the wavefronts of plane x
+// wait for the pulls of planes x - gap - 1 .. x - gap + 1, which should be
+// long done -- twice the planes that the blocks resident at once cover, plus
+// two.  At most nx (then every plane is written into a free slot).
+inline uint32_t ring_gap(uint32_t nx, uint64_t blocks_per_plane, uint64_t resident_blocks) {
+    if (blocks_per_plane == 0) return nx;
+    const uint64_t planes = (resident_blocks + blocks_per_plane - 1) / blocks_per_plane;
+    const uint64_t g = 2 * planes + 2;
+    return g < nx ? (uint32_t)g : nx;
+}
+
 }  // namespace d3q19

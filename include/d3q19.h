@@ -194,8 +194,15 @@ size_t d3q19_device_bytes(const d3q19_lattice* lat);
  * post-collision populations of the last step (f^n in place after a load).
  * One fused pass per step: each pulled population is decoded from its
  * source cell's codes (row i of M^-1, 3-5 codes), the cell collides in FP32
- * and its 19 codes are stored.  Device memory: 2 x 38 B per cell.  The
- * ranges must cover the moments of every stored state: calibrate them with
+ * and its 19 codes are stored.  Device memory: on lattices with rows of 64
+ * cells or more, one ring of nx + gap x planes of codes (38 B per cell and
+ * a gap of a few planes: about 40 B per cell at 256^3) -- the step writes
+ * each plane of the new state into the slot of a plane gap planes back, in
+ * place, once the pulls from that plane are consumed; the environment
+ * variable D3Q19_RING_GAP, read by d3q19_create, sets a smaller gap (a test
+ * knob, at least 2).  A wait that gives up (it should not) makes the next
+ * synchronising call return D3Q19_EHIP.  Shorter rows: 2 x 38 B per cell.
+ * The ranges must cover the moments of every stored state: calibrate them with
  * d3q19_moment_range_stored.  Storage D3Q19_ME16_REF (above) keeps the
  * variant's own quantization point (f^n) and FP64 arithmetic instead. */
 /* M (moments = M f, row-major 19 x 19, entries 0/+-1) and 4 M^-1 (integers). */

@@ -1,0 +1,105 @@
+"""GPU checks of the moment-coded storage's single ring of planes.
+
+An "me16" lattice on the row grid keeps its codes in one ring of nx + gap x
+planes: a step writes plane x of the new state into the slot of plane
+x - gap of the old one, in the same launch, once the wavefronts pulling from
+that plane have consumed their codes (per-plane counters; d3q19_kernels.hpp:
+me_ring_sync).  The order in which wavefronts run then must not matter: a
+step in one launch -- with the gap the library chooses, or a gap of 2 or 5
+planes, where wavefronts do wait for each other -- must give the codes of
+the same step launched one plane at a time (strictly ordered), bit for bit,
+in every boundary mode; D3Q19_RING_GAP and D3Q19_STEP_CHUNKS (read when a
+lattice is created) set the gap and the launches per step.
+"""
+
+from __future__ import annotations
+
+import os
+
+import numpy as np
+import pytest
+
+from conftest import print_case
+
+pytestmark = pytest.mark.gpu
+
+TAU = 0.6
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import lattice_boltzmann_d3q19_amd as m
+
+    return m
+
+
+def created(amd, shape, env, **kw):
+    saved = {k: os.environ.get(k) for k in ("D3Q19_RING_GAP", "D3Q19_STEP_CHUNKS")}
+    try:
+        for k in saved:
+            os.environ.pop(k, None)
+        os.environ.update(env)
+        return amd.Lattice(shape, TAU, storage="me16", **kw)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+CASES = {
+    "periodic": dict(),
+    "forced-ywalls": dict(boundary="ywalls", force=(1e-5, 0.0, 2e-6), wall_velocity_top=(0.03, 0.0, 0.0)),
+    "cavity": dict(boundary="halfway", wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                   face_velocity={"+z": (0.05, 0.0, 0.0)}),
+}
+
+
+@pytest.fixture(scope="module")
+def initial():
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+
+    shape = (96, 128, 128)
+    return shape, random_solenoidal(shape, 0.02, seed=5)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_ring_step_in_one_launch_matches_plane_by_plane(amd, initial, case):
+    from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+    shape, u0 = initial
+    steps = 12
+    kw = CASES[case]
+    lo, hi = calibrate_moment_ranges(shape, TAU, steps, u=u0, **kw)
+    runs = {}
+    for name, env in (("planes", {"D3Q19_STEP_CHUNKS": str(shape[0])}), ("auto", {}),
+                      ("gap2", {"D3Q19_RING_GAP": "2"}), ("gap5", {"D3Q19_RING_GAP": "5"})):
+        print_case("me16-ring", case=case, run=name, shape=shape, steps=steps)
+        lat = created(amd, shape, env, **kw)
+        lat.set_moment_ranges(lo, hi).set_equilibrium(None, u0).step(steps)
+        runs[name] = (lat.raw_state()[0].copy(), lat.diagnostics(), lat.device_bytes)
+        lat.close()
+    ref_state, ref_diag, _ = runs["planes"]
+    for name in ("auto", "gap2", "gap5"):
+        state, diag, _ = runs[name]
+        assert np.array_equal(state, ref_state), name
+        assert diag == ref_diag, name
+    # a tighter gap holds fewer planes
+    assert runs["gap2"][2] < runs["gap5"][2] < runs["auto"][2]
+
+
+def test_ring_holds_about_half_the_memory_of_fp16(amd):
+    """256^3: the codes of one state and a gap of a few planes, against two
+    FP16 lattices (38 B per cell each)."""
+    shape = (256, 256, 256)
+    print_case("me16-ring-memory", shape=shape)
+    me = created(amd, shape, {})
+    f16 = amd.Lattice(shape, TAU, storage="f16")
+    ratio = me.device_bytes / f16.device_bytes
+    per_cell = me.device_bytes / np.prod(shape)
+    me.close()
+    f16.close()
+    print(f"me16 ring: {per_cell:.1f} B per cell, {ratio:.3f} of FP16", flush=True)
+    assert ratio < 0.56
+    assert per_cell < 42.0
