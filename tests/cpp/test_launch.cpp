@@ -1,7 +1,7 @@
 // Host-only checks of the launch-shape arithmetic (d3q19_launch.hpp): a
 // reservation of reserve_lds(L, k) bytes per block leaves room for exactly k
 // blocks on a CU with L bytes of LDS, and stays within 64 KiB per block; the
-// plane chunks of a fused step.
+// plane chunks of a fused step; the ring gap of the moment-coded step.
 #include <cstdio>
 #include <initializer_list>
 
@@ -71,6 +71,24 @@ int main() {
                 std::printf("chunk_planes(%u, %u) = %u covers %u planes in %u launches\n", nx, n, S, covered, launches);
                 ++bad;
             }
+        }
+    }
+    // ring gap of the moment-coded step: twice the planes the resident blocks
+    // cover plus two, at most nx
+    struct Gap { unsigned nx; unsigned long long bpp, resident; unsigned want; };
+    const Gap gaps[] = {
+        {256, 128, 1024, 18},   // 256^3: 128 blocks per plane, 256 CUs x 4 blocks
+        {96, 32, 1024, 66},     // 96 x 128 x 128
+        {512, 512, 1024, 6},    // 512^3
+        {64, 2, 1024, 64},      // more resident blocks than the lattice holds: nx (two-lattice footprint)
+        {10, 0, 1024, 10},      // no blocks per plane: nx
+        {3, 1000, 1, 3},        // clamp at nx
+    };
+    for (const Gap& g : gaps) {
+        const unsigned v = d3q19::ring_gap(g.nx, g.bpp, g.resident);
+        if (v != g.want) {
+            std::printf("ring_gap(%u, %llu, %llu) = %u, want %u\n", g.nx, g.bpp, g.resident, v, g.want);
+            ++bad;
         }
     }
     std::printf(bad ? "FAIL\n" : "OK\n");
