@@ -112,12 +112,16 @@ struct d3q19_lattice {
     void* ring_saved = nullptr;     // plane 0 of the state a step reads
     uint32_t* ring_done = nullptr;  // nx counters, ME_RING_STRIDE apart
     uint32_t* ring_err = nullptr;   // set by a wait that gave up
+    bool ring_fault_test = false;   // D3Q19_RING_FAULT_TEST: every wait gives up at once
     int64_t steps = 0;
     size_t bytes = 0;
     uint32_t lds_per_cu = 0;  // LDS bytes per CU of the device, 0 if unknown
 };
 
 namespace {
+
+// Polls of a short count before a ring step's wait gives up (about 10 ms).
+constexpr uint32_t ME_RING_MAX_SPINS = 40000;
 
 template <class C>
 Params<C> make_params(const d3q19_lattice* L) {
@@ -263,6 +267,11 @@ int launch_step_l(d3q19_lattice* L, int32_t n, float* rho, float* u) {
                 q.ring_out = (L->ring_base + L->ring_np - L->ring_gap) % L->ring_np;
                 q.ring_gap = L->ring_gap;
                 q.ring_target = L->ring_blocks * (L->ring_epoch + 1u);
+                q.ring_spins = ME_RING_MAX_SPINS;
+                if (L->ring_fault_test) {  // a count no block reaches, one poll: the failure path
+                    q.ring_target += 0x40000000u;
+                    q.ring_spins = 1u;
+                }
                 q.ring_saved = (const char*)L->ring_saved;
                 q.ring_done = L->ring_done;
                 q.ring_err = L->ring_err;
@@ -894,6 +903,8 @@ int d3q19_create(const d3q19_config* cfg, void* stream, d3q19_lattice** out) {
             const long g = std::strtol(gap, nullptr, 10);
             if (g >= 2 && g < (long)L->ring_gap) L->ring_gap = (uint32_t)g;
         }
+        if (const char* ft = std::getenv("D3Q19_RING_FAULT_TEST"))  // a test knob: the waits' failure path
+            L->ring_fault_test = std::strtol(ft, nullptr, 10) == 1;
         L->ring_np = (uint32_t)cfg->nx + L->ring_gap;
         L->plane_bytes = (size_t)cfg->ny * L->row_bytes;
         L->ring_blocks = sf.grid.x * sf.grid.y;

@@ -267,6 +267,7 @@ template <class C> struct Params {
     const char* ring_saved;  // step: plane 0 of the state read, copied before the step
     uint32_t* ring_done;     // step: per plane, wavefronts whose pulls are consumed (monotonic)
     uint32_t* ring_err;      // step: set to 1 when a wait gave up
+    uint32_t ring_spins;     // step: polls of a short count before a wait gives up
 };
 
 // Fill the z-wall terms of p from its face velocities (host).

@@ -39,7 +39,6 @@ __device__ __forceinline__ void step_moments_out(float* rho_out, float* u_out, u
 // per block on packed counters with every wavefront reading them 8.8 -- the
 // operations on one counter line serialise; a counter per line with one
 // wavefront per block reading them when it got there, 41.8.
-constexpr uint32_t ME_RING_MAX_SPINS = 40000;
 constexpr uint32_t ME_RING_STRIDE = 64;  // uint32 per counter: one 256-byte line each
 
 __device__ __forceinline__ bool me_ring_short(uint32_t count, uint32_t target) {
@@ -78,7 +77,7 @@ __device__ __forceinline__ void me_ring_sync(const Params<float>& p, uint32_t x,
                     const uint32_t* c = p.ring_done + (size_t)(x - p.ring_gap - 1 + k) * ME_RING_STRIDE;
                     uint32_t seen = dep.c[k], spins = 0;
                     while (me_ring_short(seen, p.ring_target)) {
-                        if (++spins > ME_RING_MAX_SPINS) {
+                        if (++spins > p.ring_spins) {
                             __hip_atomic_store(p.ring_err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             break;
                         }
@@ -90,7 +89,7 @@ __device__ __forceinline__ void me_ring_sync(const Params<float>& p, uint32_t x,
             } else {
                 uint32_t spins = 0;
                 while (__hip_atomic_load(lds + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) == 0 &&
-                       ++spins < 4 * ME_RING_MAX_SPINS)
+                       ++spins < 4 * p.ring_spins + 4)
                     __builtin_amdgcn_s_sleep(2);
             }
         }

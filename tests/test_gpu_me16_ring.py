@@ -34,7 +34,7 @@ def amd():
 
 
 def created(amd, shape, env, **kw):
-    saved = {k: os.environ.get(k) for k in ("D3Q19_RING_GAP", "D3Q19_STEP_CHUNKS")}
+    saved = {k: os.environ.get(k) for k in ("D3Q19_RING_GAP", "D3Q19_STEP_CHUNKS", "D3Q19_RING_FAULT_TEST")}
     try:
         for k in saved:
             os.environ.pop(k, None)
@@ -103,3 +103,27 @@ def test_ring_holds_about_half_the_memory_of_fp16(amd):
     print(f"me16 ring: {per_cell:.1f} B per cell, {ratio:.3f} of FP16", flush=True)
     assert ratio < 0.56
     assert per_cell < 42.0
+
+
+def test_a_wait_that_gives_up_is_reported(amd, initial):
+    """The waits' failure path, forced from the host: with
+    D3Q19_RING_FAULT_TEST=1 every wait polls, once, a count that no block
+    reaches and gives up, so every kernel still ends; the next synchronising
+    call reports D3Q19_EHIP, and keeps reporting it."""
+    from lattice_boltzmann_d3q19_amd._lib import D3Q19_EHIP, D3Q19Error
+    from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+    shape, u0 = initial
+    print_case("me16-ring-wait-gives-up", shape=shape)
+    lo, hi = calibrate_moment_ranges(shape, TAU, 4, u=u0)
+    lat = created(amd, shape, {"D3Q19_RING_FAULT_TEST": "1"})
+    lat.set_moment_ranges(lo, hi).set_equilibrium(None, u0)
+    lat.step(1)
+    lat.diagnostics()  # the first step after a load collides in place: no waits
+    lat.step(2)
+    for read in (lat.diagnostics, lat.populations):
+        with pytest.raises(D3Q19Error) as e:
+            read()
+        assert e.value.status == D3Q19_EHIP
+        assert "waited too long" in str(e.value)
+    lat.close()
