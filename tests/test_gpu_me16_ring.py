@@ -89,6 +89,48 @@ def test_ring_step_in_one_launch_matches_plane_by_plane(amd, initial, case):
     assert runs["gap2"][2] < runs["gap5"][2] < runs["auto"][2]
 
 
+RAGGED = {
+    # nz = 100: rows padded to 128, blocks of 128 z x 4 rows; ny = 37: the last block has one live row
+    "padded-rows-odd-ny": (40, 37, 100),
+    # nx = 3: the automatic gap covers the whole lattice (no waits); gap 2 still writes in place
+    "three-planes": (3, 20, 64),
+    # nz = 192: three z segments of 64 per row, blocks of 8 rows of which 5 are live
+    "short-blocks": (17, 5, 192),
+}
+
+
+@pytest.mark.parametrize("bc", ["forced", "cavity"])
+@pytest.mark.parametrize("name", list(RAGGED))
+def test_ring_on_ragged_lattices(amd, name, bc):
+    """Partly live blocks (padded rows, ny not a multiple of the block's
+    rows), a gap as large as the lattice, and plane chunks over a tight gap:
+    the ring step equals the plane-by-plane step bit for bit."""
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+    from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+    shape = RAGGED[name]
+    u0 = random_solenoidal(shape, 0.02, seed=11)
+    kw = (dict(force=(2e-6, -1e-6, 3e-6)) if bc == "forced" else
+          dict(boundary="halfway", wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+               face_velocity={"+z": (0.05, 0.0, 0.0)}))
+    steps = 9
+    lo, hi = calibrate_moment_ranges(shape, TAU, steps, u=u0, **kw)
+    runs = {}
+    for run, env in (("planes", {"D3Q19_STEP_CHUNKS": str(shape[0])}), ("auto", {}), ("gap2", {"D3Q19_RING_GAP": "2"}),
+                     ("gap2-chunks3", {"D3Q19_RING_GAP": "2", "D3Q19_STEP_CHUNKS": "3"})):
+        print_case("me16-ring-ragged", shape=shape, bc=bc, run=run, steps=steps)
+        lat = created(amd, shape, env, **kw)
+        lat.set_moment_ranges(lo, hi).set_equilibrium(None, u0).step(steps)
+        runs[run] = (lat.raw_state()[0].copy(), lat.populations(), lat.diagnostics())
+        lat.close()
+    ref = runs["planes"]
+    assert np.isfinite(ref[1]).all()
+    for run in ("auto", "gap2", "gap2-chunks3"):
+        assert np.array_equal(runs[run][0], ref[0]), run
+        assert np.array_equal(runs[run][1], ref[1]), run
+        assert runs[run][2] == ref[2], run
+
+
 def test_ring_holds_about_half_the_memory_of_fp16(amd):
     """256^3: the codes of one state and a gap of a few planes, against two
     FP16 lattices (38 B per cell each)."""
