@@ -19,9 +19,12 @@
 //                y-walls: S^T = P^T B^T, B^T applied on the wall rows first
 //   dL/dF, dL/ds accumulate the parameter parts of each step's Jacobian
 //   dL/du0     = dfeq(1,u0)/du0^T lambda^0
-// States between checkpoints (every K steps) are recomputed by the forward
-// kernel itself, writing straight into the segment slots.  Checkpoints may be
-// stored as FP16 (f - w) and expanded to the working precision on use.
+// The states between checkpoints (every K steps) are recomputed by the
+// forward kernel itself, which also stores rho - 1 and the momentum sum m of
+// each pulled f^n (k_step with MOM): J_C(f^n) depends on f^n only through
+// them, so the reverse step reads 4 values per cell instead of pulling 19,
+// and a segment keeps K moment slots instead of K - 1 states.  Checkpoints
+// may be stored as FP16 (f - w) and expanded to the working precision on use.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -123,12 +126,11 @@ __global__ void __launch_bounds__(step_block<Tag>())
 }
 
 // One reverse step: lambda_out = J_C(f^n)^T S^T lambda_in; the parameter
-// parts go to gF (force) and gs (Brinkman fraction).
-// LOCAL0: f^n is stored in local layout (the y-wall chain's f^0) instead of
-// being pulled from a post-collision state.
-template <class Tag, int BC, bool BRINK, bool LOCAL0 = false>
+// parts go to gF (force) and gs (Brinkman fraction).  mom: rho - 1 and m of
+// f^n, (ncell, 4), stored by the recomputation (store_adjoint_moments).
+template <class Tag, int BC, bool BRINK>
 __global__ void __launch_bounds__(step_block<Tag>())
-    k_adj_step(const typename Codec<Tag>::raw* __restrict__ H, Params<typename Codec<Tag>::C> p,
+    k_adj_step(const typename Codec<Tag>::C* __restrict__ mom, Params<typename Codec<Tag>::C> p,
                const typename Codec<Tag>::raw* __restrict__ lam_in, typename Codec<Tag>::raw* __restrict__ lam_out,
                typename Codec<Tag>::C* __restrict__ gF, typename Codec<Tag>::C* __restrict__ gs) {
     typedef typename Codec<Tag>::C C;
@@ -153,13 +155,8 @@ __global__ void __launch_bounds__(step_block<Tag>())
                                      zs[1 + CZ[i]] * ES;
         mu[i] = ld_at<Tag, true>(src, i);
     }
-    C d[Q];
-    if constexpr (LOCAL0)
-        load_local<Tag>(H, p, x, y, z, d);
-    else
-        pull<Tag, BC, true>(H, p, x, y, z, d);
-    C drho, mx, my, mz;
-    moments(d, drho, mx, my, mz);
+    const Vec4<C> m4 = ((const Vec4<C>*)mom)[cell];
+    const C drho = m4.x, mx = m4.y, my = m4.z, mz = m4.w;
     const C rho = C(1) + drho;
     const C ir = C(1) / rho;
     const C s = BRINK ? p.brink[cell] : C(0);
@@ -300,7 +297,9 @@ struct d3q19_adjoint {
     uint32_t ncell = 0, nzp = 0;
     size_t es = 0, state_bytes = 0, ckpt_bytes = 0;
     std::vector<void*> ckpt;  // H_{kK}
-    std::vector<void*> seg;   // working states: recompute slots, also the forward ping-pong
+    std::vector<void*> seg;   // working states: the forward's and the recomputation's ping-pong
+    void* moms = nullptr;     // K moment slots of a segment, mom_bytes each (store_adjoint_moments)
+    size_t mom_bytes = 0;
     void* lam[2] = {nullptr, nullptr};
     void* u0 = nullptr;
     void* force = nullptr;
@@ -353,13 +352,12 @@ void adj_grid(const d3q19_adjoint* A, dim3& grid, dim3& block, uint32_t threads 
     grid = dim3((nz + bz - 1) / bz, (A->cfg.ny + by - 1) / by, A->cfg.nx);
 }
 
-// One forward step H_{j+1} = C(S H_j).  first: j = 0 of a y-wall chain,
-// whose H_0 is f^0 itself, so the step is the local collision H_1 = C(f^0).
-template <class Tag>
-int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
+template <class Tag, bool MOM>
+int step_into_t(d3q19_adjoint* A, void* in, void* out, bool first, void* mom) {
     typedef typename Codec<Tag>::C C;
     typedef typename Codec<Tag>::raw R;
-    const Params<C> p = adj_params<C>(A, A->es);
+    Params<C> p = adj_params<C>(A, A->es);
+    p.mom = (C*)mom;
     dim3 g, b, gf, bf;
     adj_grid<Tag>(A, g, b);                       // k_collide_local
     adj_grid<Tag>(A, gf, bf, fused_block<Tag>());  // k_step
@@ -370,20 +368,30 @@ int step_into(d3q19_adjoint* A, void* in, void* out, bool first) {
     const uint32_t lds = reserve_lds(A->lds_per_cu, step_blocks_per_cu<Tag>());
     switch (A->cfg.boundary) {
         case D3Q19_BC_HALFWAY:
-            k_step<Tag, BC_HALFWAY, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_HALFWAY, true, true, MOM><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         case D3Q19_BC_YWALLS:
             if (first)
-                k_collide_local<Tag, true><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+                k_collide_local<Tag, true, MOM><<<g, b, 0, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             else
-                k_step<Tag, BC_YWALLS, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+                k_step<Tag, BC_YWALLS, true, true, MOM><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
             break;
         default:
-            k_step<Tag, BC_PERIODIC, true><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
+            k_step<Tag, BC_PERIODIC, true, true, MOM><<<gf, bf, lds, A->stream>>>(src, (R*)out, p, nullptr, nullptr);
     }
     D3Q19_HIPCHK(hipGetLastError());
     return D3Q19_OK;
 }
+
+// One forward step H_{j+1} = C(S H_j).  first: j = 0 of a y-wall chain,
+// whose H_0 is f^0 itself, so the step is the local collision H_1 = C(f^0).
+// mom: where the step also stores the moments of f^j (the recomputation), or null.
+template <class Tag>
+int step_into(d3q19_adjoint* A, void* in, void* out, bool first, void* mom = nullptr) {
+    if (mom) return step_into_t<Tag, true>(A, in, out, first, mom);
+    return step_into_t<Tag, false>(A, in, out, first, nullptr);
+}
+
 
 // Working state <-> FP16 checkpoint.
 template <class Tag>
@@ -447,6 +455,9 @@ int run_forward(d3q19_adjoint* A, void* u_out_dev, int out_dt) {
     }
 }
 
+// Moment slot j of a segment (rho - 1 and m of f^{n0 + j}, (ncell, 4)).
+void* mom_slot(const d3q19_adjoint* A, int32_t j) { return (char*)A->moms + (size_t)j * A->mom_bytes; }
+
 template <class Tag, int BC, bool BRINK>
 int run_backward_t(d3q19_adjoint* A) {
     typedef typename Codec<Tag>::C C;
@@ -461,26 +472,26 @@ int run_backward_t(d3q19_adjoint* A) {
     for (int32_t k = A->nck - 1; k >= 0; --k) {
         const int32_t n0 = k * A->K;
         const int32_t len = std::min(A->K, A->T - n0);
-        // slot j holds H_{n0 + j}: the checkpoint itself (same storage) or its
-        // expansion into seg[0]; the rest recomputed into seg[]
-        const int off = A->ckpt_same ? 1 : 0;
-        auto slot = [&](int32_t j) -> void* { return (j == 0 && A->ckpt_same) ? A->ckpt[k] : A->seg[j - off]; };
-        if (!A->ckpt_same)
+        // recompute H_{n0 + 1} .. H_{n0 + len} from the checkpoint through the
+        // two working states, storing the moments of f^{n0 + j} in slot j
+        const void* h = A->ckpt[k];
+        if (!A->ckpt_same) {
             if (int e = convert<Tag>(A, A->ckpt[k], A->seg[0], false)) return e;
-        for (int32_t j = 1; j < len; ++j)
-            if (int e = step_into<Tag>(A, slot(j - 1), slot(j), k == 0 && j == 1)) return e;
+            h = A->seg[0];
+        }
+        for (int32_t j = 0; j < len; ++j) {
+            void* out = (h == A->seg[0]) ? A->seg[1] : A->seg[0];
+            if (int e = step_into<Tag>(A, const_cast<void*>(h), out, k == 0 && j == 0, mom_slot(A, j))) return e;
+            h = out;
+        }
         for (int32_t j = len - 1; j >= 0; --j) {
             if constexpr (BC == BC_YWALLS) {
                 const uint32_t n = p.nx * (p.ny > 1 ? 2u : 1u) * p.nz;
                 k_adj_ywalls_T<Tag><<<(n + 255) / 256, 256, 0, A->stream>>>((R*)A->lam[cur], p);
                 D3Q19_HIPCHK(hipGetLastError());
             }
-            if (BC == BC_YWALLS && k == 0 && j == 0)
-                k_adj_step<Tag, BC, BRINK, true><<<g, b, 0, A->stream>>>(
-                    (const R*)slot(j), p, (const R*)A->lam[cur], (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
-            else
-                k_adj_step<Tag, BC, BRINK, false><<<g, b, 0, A->stream>>>(
-                    (const R*)slot(j), p, (const R*)A->lam[cur], (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
+            k_adj_step<Tag, BC, BRINK><<<g, b, 0, A->stream>>>((const C*)mom_slot(A, j), p, (const R*)A->lam[cur],
+                                                                (R*)A->lam[cur ^ 1], (C*)A->gF, (C*)A->gs);
             D3Q19_HIPCHK(hipGetLastError());
             cur ^= 1;
         }
@@ -507,18 +518,18 @@ void free_all(d3q19_adjoint* A) {
         if (p) (void)hipFree(p);
     for (void* p : A->seg)
         if (p) (void)hipFree(p);
-    void* ps[] = {A->lam[0], A->lam[1], A->u0, A->force, A->s, A->gF, A->gu, A->gs};
+    void* ps[] = {A->moms, A->lam[0], A->lam[1], A->u0, A->force, A->s, A->gF, A->gu, A->gs};
     for (void* p : ps)
         if (p) (void)hipFree(p);
 }
 
-// Device bytes of the adjoint with checkpoint interval K.
+// Device bytes of the adjoint with checkpoint interval K: the checkpoints, K
+// moment slots, two working states, two adjoint states and the fields.
 size_t plan_bytes(const d3q19_adjoint* A, int32_t K) {
     const size_t nck = (size_t)((A->T + K - 1) / K);
-    const size_t nseg = (size_t)std::max(2, A->ckpt_same ? K - 1 : K);
     const size_t vec = 3 * (size_t)A->ncell * A->es;
     const size_t scal = (size_t)A->ncell * A->es;
-    return nck * A->ckpt_bytes + nseg * A->state_bytes + 2 * A->state_bytes + 4 * vec +
+    return nck * A->ckpt_bytes + (size_t)K * A->mom_bytes + 4 * A->state_bytes + 4 * vec +
            (A->brink ? 2 * scal : 0);
 }
 
@@ -573,6 +584,7 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     const size_t rows = (size_t)cfg->nx * cfg->ny;
     A->state_bytes = rows * Q * A->nzp * A->es;
     A->ckpt_bytes = rows * Q * A->nzp * (A->ckpt_same ? A->es : 2);
+    A->mom_bytes = 4 * (size_t)ncell * A->es;
     int32_t K = opt->checkpoint_every;
     if (K <= 0) {
         K = 1;
@@ -594,9 +606,10 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     const size_t scal = (size_t)ncell * A->es;
     bool ok = true;
     A->ckpt.assign(A->nck, nullptr);
-    A->seg.assign(std::max(2, A->ckpt_same ? K - 1 : K), nullptr);
+    A->seg.assign(2, nullptr);
     for (auto& p : A->ckpt) ok = ok && hipMalloc(&p, A->ckpt_bytes) == hipSuccess;
     for (auto& p : A->seg) ok = ok && hipMalloc(&p, A->state_bytes) == hipSuccess;
+    ok = ok && hipMalloc(&A->moms, (size_t)K * A->mom_bytes) == hipSuccess;
     ok = ok && hipMalloc(&A->lam[0], A->state_bytes) == hipSuccess &&
          hipMalloc(&A->lam[1], A->state_bytes) == hipSuccess;
     ok = ok && hipMalloc(&A->u0, vec) == hipSuccess && hipMalloc(&A->force, vec) == hipSuccess;

@@ -268,6 +268,9 @@ template <class C> struct Params {
     uint32_t* ring_done;     // step: per plane, wavefronts whose pulls are consumed (monotonic)
     uint32_t* ring_err;      // step: set to 1 when a wait gave up
     uint32_t ring_spins;     // step: polls of a short count before a wait gives up
+    // The adjoint's recomputation (k_step / k_collide_local with MOM): rho - 1
+    // and the momentum sum m of the pulled f^n, (ncell, 4) in the compute type.
+    C* mom;
 };
 
 // Fill the z-wall terms of p from its face velocities (host).

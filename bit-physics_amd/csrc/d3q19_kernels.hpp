@@ -97,12 +97,25 @@ __device__ __forceinline__ void me_ring_sync(const Params<float>& p, uint32_t x,
     asm volatile("" ::: "memory");  // no store of this wavefront above the wait
 }
 
+// The adjoint's recomputation: rho - 1 and m of the pulled f^n of a cell, the
+// only part of the state its reverse step reads (k_adj_step), computed as the
+// reverse step computed them from the pulled state; one 4-vector per cell.
+template <class C>
+using Vec4 = typename std::conditional<std::is_same<C, double>::value, double4, float4>::type;
+template <class C>
+__device__ __forceinline__ void store_adjoint_moments(const Params<C>& p, uint32_t cell, const C d[Q]) {
+    Vec4<C> m;
+    moments(d, m.x, m.y, m.z, m.w);
+    ((Vec4<C>*)p.mom)[cell] = m;
+}
+
 // ---------------------------------------------------------------------------
 // The hot path: one fused pull-stream / boundary / moments / BGK+Guo pass.
 // Reads the post-collision state of step n-1, writes that of step n, and
 // (optionally) rho and u of the reference state f^n from the same pass.
 // ---------------------------------------------------------------------------
-template <class Tag, int BC, bool FORCED, bool ROWS = true>
+// MOM: also store the moments of the pulled f^n (store_adjoint_moments).
+template <class Tag, int BC, bool FORCED, bool ROWS = true, bool MOM = false>
 static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag>())
     k_step(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
            Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
@@ -171,6 +184,7 @@ static __global__ void __launch_bounds__(fused_block<Tag>(), fused_min_waves<Tag
             for (int i = 0; i < Q; ++i) d[i] = C(0);
         } else {
             pull<Tag, BC, false, U>(in, p, x, y, z, d);
+            if constexpr (MOM) store_adjoint_moments(p, cell, d);
             collide<C, FORCED>(d, p, cell, rho, ux, uy, uz);
         }
     }
@@ -197,7 +211,8 @@ static __global__ void __launch_bounds__(step_block<Tag>())
 
 // First step after a load: collide the in-place f^n (no streaming) into the
 // post-collision storage the fused step consumes; optionally write rho/u of f^n.
-template <class Tag, bool ROWS = true>
+// MOM: also store the moments of f^n (store_adjoint_moments).
+template <class Tag, bool ROWS = true, bool MOM = false>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_collide_local(const typename Codec<Tag>::raw* __restrict__ in, typename Codec<Tag>::raw* __restrict__ out,
                     Params<typename Codec<Tag>::C> p, float* rho_out, float* u_out) {
@@ -213,6 +228,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
         for (int i = 0; i < Q; ++i) d[i] = C(0);
     } else {
         load_local<Tag, U>(in, p, x, y, z, d);
+        if constexpr (MOM) store_adjoint_moments(p, cell, d);
         collide(d, p, cell, rho, ux, uy, uz);
     }
     store_local<Tag, U>(out, p, x, y, z, d);
