@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(step_block<Tag>())
                                      zs[1 + CZ[i]] * ES;
         mu[i] = ld_at<Tag, true>(src, i);
     }
-    const Vec4<C> m4 = ((const Vec4<C>*)mom)[cell];
+    const Vec4<C> m4 = __builtin_nontemporal_load((const Vec4<C>*)mom + cell);
     const C drho = m4.x, mx = m4.y, my = m4.z, mz = m4.w;
     const C rho = C(1) + drho;
     const C ir = C(1) / rho;

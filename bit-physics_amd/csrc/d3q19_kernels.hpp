@@ -99,14 +99,19 @@ __device__ __forceinline__ void me_ring_sync(const Params<float>& p, uint32_t x,
 
 // The adjoint's recomputation: rho - 1 and m of the pulled f^n of a cell, the
 // only part of the state its reverse step reads (k_adj_step), computed as the
-// reverse step computed them from the pulled state; one 4-vector per cell.
+// reverse step computed them from the pulled state; one 4-vector per cell,
+// stored nontemporally (read once, by the reverse pass; backward 264.3 ->
+// 263.1 ms at 128^3 x 2000 steps against a plain store and load).
+typedef float MomF4 __attribute__((ext_vector_type(4)));
+typedef double MomD4 __attribute__((ext_vector_type(4)));
 template <class C>
-using Vec4 = typename std::conditional<std::is_same<C, double>::value, double4, float4>::type;
+using Vec4 = typename std::conditional<std::is_same<C, double>::value, MomD4, MomF4>::type;
 template <class C>
 __device__ __forceinline__ void store_adjoint_moments(const Params<C>& p, uint32_t cell, const C d[Q]) {
-    Vec4<C> m;
-    moments(d, m.x, m.y, m.z, m.w);
-    ((Vec4<C>*)p.mom)[cell] = m;
+    C drho, mx, my, mz;
+    moments(d, drho, mx, my, mz);
+    const Vec4<C> m = {drho, mx, my, mz};
+    __builtin_nontemporal_store(m, (Vec4<C>*)p.mom + cell);
 }
 
 // ---------------------------------------------------------------------------
