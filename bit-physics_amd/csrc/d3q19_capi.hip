@@ -297,6 +297,19 @@ int launch_step(d3q19_lattice* L, int32_t n, float* rho, float* u) {
     return L->rows ? launch_step_l<Tag, true>(L, n, rho, u) : launch_step_l<Tag, false>(L, n, rho, u);
 }
 
+__global__ void k_clear_word(uint32_t* w) {
+    if (threadIdx.x == 0) *w = 0u;
+}
+
+// A loaded state replaces whatever a failed ring step left: clear its flag
+// (a kernel rather than a memset node, so that the load may be captured).
+int ring_clear(d3q19_lattice* L) {
+    if (!L->ring) return D3Q19_OK;
+    k_clear_word<<<1, 64, 0, L->stream>>>(L->ring_err);
+    HIPCHK(hipGetLastError());
+    return D3Q19_OK;
+}
+
 template <class Tag, class In>
 int launch_init(d3q19_lattice* L, const In* f_dev) {
     typedef typename Codec<Tag>::C C;
@@ -308,6 +321,7 @@ int launch_init(d3q19_lattice* L, const In* f_dev) {
     else
         k_load<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(f_dev, buf_of<Tag>(L, L->cur), p);
     HIPCHK(hipGetLastError());
+    if (int e = ring_clear(L)) return e;
     L->has_state = true;
     L->local = true;
     L->steps = 0;
@@ -326,6 +340,7 @@ int launch_init_eq(d3q19_lattice* L, const In* rho_dev, const In* u_dev) {
     else
         k_load_eq<Tag, In, false><<<sg.grid, sg.block, 0, L->stream>>>(rho_dev, u_dev, buf_of<Tag>(L, L->cur), p);
     HIPCHK(hipGetLastError());
+    if (int e = ring_clear(L)) return e;
     L->has_state = true;
     L->local = true;
     L->steps = 0;
@@ -816,11 +831,13 @@ int ring_copy(d3q19_lattice* L, void* host, hipMemcpyKind kind) {
 }
 
 // After a synchronisation: whether a ring step's wait gave up (its result
-// would be unreliable).  The flag stays set.
+// would be unreliable).  The flag stays set until a state is loaded
+// (ring_clear).
 int ring_check(d3q19_lattice* L) {
     if (!L->ring) return D3Q19_OK;
     uint32_t err = 0;
-    HIPCHK(hipMemcpy(&err, L->ring_err, sizeof(err), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpyAsync(&err, L->ring_err, sizeof(err), hipMemcpyDeviceToHost, L->stream));
+    HIPCHK(hipStreamSynchronize(L->stream));
     if (err)
         return fail(D3Q19_EHIP, "a moment-coded step waited too long for the pulls of the planes it overwrites; "
                                 "the lattice state is unreliable");
@@ -1347,6 +1364,7 @@ int d3q19_write_state(d3q19_lattice* L, const void* src, int32_t local, int64_t 
         return fail(D3Q19_ESTATE, "a moment-encoded lattice needs d3q19_set_moment_ranges first");
     if (L->ring) {
         if (int e = ring_copy(L, const_cast<void*>(src), hipMemcpyHostToDevice)) return e;
+        if (int e = ring_clear(L)) return e;
     } else {
         void* dst = L->me_ref ? L->buf[0] : L->buf[L->cur];
         HIPCHK(hipMemcpyAsync(dst, src, d3q19_state_bytes(L), hipMemcpyHostToDevice, L->stream));

@@ -200,9 +200,11 @@ size_t d3q19_device_bytes(const d3q19_lattice* lat);
  * each plane of the new state into the slot of a plane gap planes back, in
  * place, once the pulls from that plane are consumed; the environment
  * variable D3Q19_RING_GAP, read by d3q19_create, sets a smaller gap (a test
- * knob, at least 2).  A wait that gives up (it should not) makes the next
- * synchronising call return D3Q19_EHIP; D3Q19_RING_FAULT_TEST=1 (a test knob)
- * makes every wait give up at once.  Shorter rows: 2 x 38 B per cell.
+ * knob, at least 2).  A wait that gives up (it should not) makes every
+ * synchronising call return D3Q19_EHIP until a state is loaded again
+ * (set_populations, set_equilibrium, write_state); D3Q19_RING_FAULT_TEST=1
+ * (a test knob) makes every wait give up at once.  Shorter rows: 2 x 38 B
+ * per cell.
  * The ranges must cover the moments of every stored state: calibrate them with
  * d3q19_moment_range_stored.  Storage D3Q19_ME16_REF (above) keeps the
  * variant's own quantization point (f^n) and FP64 arithmetic instead. */

@@ -147,11 +147,13 @@ def test_ring_holds_about_half_the_memory_of_fp16(amd):
     assert per_cell < 42.0
 
 
-def test_a_wait_that_gives_up_is_reported(amd, initial):
+def test_a_wait_that_gives_up_is_reported(amd, initial, tmp_path):
     """The waits' failure path, forced from the host: with
     D3Q19_RING_FAULT_TEST=1 every wait polls, once, a count that no block
     reaches and gives up, so every kernel still ends; the next synchronising
-    call reports D3Q19_EHIP, and keeps reporting it."""
+    call reports D3Q19_EHIP, and keeps reporting it until a state is loaded:
+    a checkpoint restore or a new equilibrium makes the lattice readable again
+    (its first step collides in place, without waits)."""
     from lattice_boltzmann_d3q19_amd._lib import D3Q19_EHIP, D3Q19Error
     from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
 
@@ -161,11 +163,19 @@ def test_a_wait_that_gives_up_is_reported(amd, initial):
     lat = created(amd, shape, {"D3Q19_RING_FAULT_TEST": "1"})
     lat.set_moment_ranges(lo, hi).set_equilibrium(None, u0)
     lat.step(1)
-    lat.diagnostics()  # the first step after a load collides in place: no waits
+    good = lat.populations()  # the first step after a load collides in place: no waits
+    ck = lat.checkpoint(tmp_path / "ring.npz")
     lat.step(2)
     for read in (lat.diagnostics, lat.populations):
         with pytest.raises(D3Q19Error) as e:
             read()
         assert e.value.status == D3Q19_EHIP
         assert "waited too long" in str(e.value)
+    lat.restore(ck)
+    assert np.array_equal(lat.populations(), good)
+    lat.step(1)  # a fused step from the restored state: its waits give up again
+    with pytest.raises(D3Q19Error):
+        lat.diagnostics()
+    lat.set_equilibrium(None, u0)
+    lat.step(1).diagnostics()
     lat.close()
