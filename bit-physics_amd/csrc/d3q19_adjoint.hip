@@ -523,14 +523,17 @@ void free_all(d3q19_adjoint* A) {
         if (p) (void)hipFree(p);
 }
 
-// Device bytes of the adjoint with checkpoint interval K: the checkpoints, K
-// moment slots, two working states, two adjoint states and the fields.
-size_t plan_bytes(const d3q19_adjoint* A, int32_t K) {
-    const size_t nck = (size_t)((A->T + K - 1) / K);
+// Per-cell fields of the adjoint: u0, force, dL/dF, dL/du (3 each), and the
+// Brinkman fraction with its gradient.
+size_t field_bytes(const d3q19_adjoint* A) {
     const size_t vec = 3 * (size_t)A->ncell * A->es;
     const size_t scal = (size_t)A->ncell * A->es;
-    return nck * A->ckpt_bytes + (size_t)K * A->mom_bytes + 4 * A->state_bytes + 4 * vec +
-           (A->brink ? 2 * scal : 0);
+    return 4 * vec + (A->brink ? 2 * scal : 0);
+}
+
+// Device bytes of the adjoint with checkpoint interval K (adjoint_plan_bytes).
+size_t plan_bytes(const d3q19_adjoint* A, int32_t K) {
+    return adjoint_plan_bytes(A->T, K, A->ckpt_bytes, A->mom_bytes, A->state_bytes, field_bytes(A));
 }
 
 }  // namespace
@@ -586,11 +589,7 @@ int d3q19_adjoint_create(const d3q19_config* cfg, const d3q19_adjoint_options* o
     A->ckpt_bytes = rows * Q * A->nzp * (A->ckpt_same ? A->es : 2);
     A->mom_bytes = 4 * (size_t)ncell * A->es;
     int32_t K = opt->checkpoint_every;
-    if (K <= 0) {
-        K = 1;
-        for (int32_t k = 2; k <= A->T; ++k)
-            if (plan_bytes(A, k) < plan_bytes(A, K)) K = k;
-    }
+    if (K <= 0) K = adjoint_least_memory_K(A->T, A->ckpt_bytes, A->mom_bytes, A->state_bytes, field_bytes(A));
     K = std::min(K, A->T);
     if (opt->memory_budget && plan_bytes(A, K) > opt->memory_budget) {
         const size_t need = plan_bytes(A, K);

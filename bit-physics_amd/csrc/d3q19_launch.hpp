@@ -1,4 +1,4 @@
-// d3q19_launch.hpp -- launch-shape arithmetic of the host code.  No HIP
+// d3q19_launch.hpp -- launch-shape and memory-plan arithmetic of the host code.  No HIP
 // dependency, so host-only tests compile it with a plain C++ compiler.
 #pragma once
 
@@ -49,6 +49,33 @@ inline uint32_t ring_gap(uint32_t nx, uint64_t blocks_per_plane, uint64_t reside
     const uint64_t planes = (resident_blocks + blocks_per_plane - 1) / blocks_per_plane;
     const uint64_t g = 2 * planes + 2;
     return g < nx ? (uint32_t)g : nx;
+}
+
+// Device bytes of the adjoint with checkpoint interval K over T steps:
+// ceil(T / K) checkpoints of ckpt_bytes, K moment slots of mom_bytes (rho - 1
+// and m of each state a segment recomputes), four working states of
+// state_bytes (the forward's and the recomputation's two, the two adjoint
+// states) and `fields` bytes of per-cell fields.
+inline uint64_t adjoint_plan_bytes(int32_t T, int32_t K, uint64_t ckpt_bytes, uint64_t mom_bytes, uint64_t state_bytes,
+                                   uint64_t fields) {
+    const uint64_t nck = (uint64_t)((T + K - 1) / K);
+    return nck * ckpt_bytes + (uint64_t)K * mom_bytes + 4 * state_bytes + fields;
+}
+
+// The checkpoint interval in 1 .. T that needs the least memory (the smallest
+// such K).
+inline int32_t adjoint_least_memory_K(int32_t T, uint64_t ckpt_bytes, uint64_t mom_bytes, uint64_t state_bytes,
+                                      uint64_t fields) {
+    int32_t K = 1;
+    uint64_t best = adjoint_plan_bytes(T, 1, ckpt_bytes, mom_bytes, state_bytes, fields);
+    for (int32_t k = 2; k <= T; ++k) {
+        const uint64_t b = adjoint_plan_bytes(T, k, ckpt_bytes, mom_bytes, state_bytes, fields);
+        if (b < best) {
+            best = b;
+            K = k;
+        }
+    }
+    return K;
 }
 
 }  // namespace d3q19

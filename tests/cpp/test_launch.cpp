@@ -1,7 +1,8 @@
 // Host-only checks of the launch-shape arithmetic (d3q19_launch.hpp): a
 // reservation of reserve_lds(L, k) bytes per block leaves room for exactly k
 // blocks on a CU with L bytes of LDS, and stays within 64 KiB per block; the
-// plane chunks of a fused step; the ring gap of the moment-coded step.
+// plane chunks of a fused step; the ring gap of the moment-coded step; the
+// adjoint's memory plan and its automatic checkpoint interval.
 #include <cstdio>
 #include <initializer_list>
 
@@ -88,6 +89,38 @@ int main() {
         const unsigned v = d3q19::ring_gap(g.nx, g.bpp, g.resident);
         if (v != g.want) {
             std::printf("ring_gap(%u, %llu, %llu) = %u, want %u\n", g.nx, g.bpp, g.resident, v, g.want);
+            ++bad;
+        }
+    }
+    // the adjoint's memory plan: ceil(T / K) checkpoints, K moment slots, four
+    // working states and the fields; the automatic K needs the least of it
+    struct Plan { int T; unsigned long long ck, mom, st, fields; int wantK; unsigned long long wantBytes; };
+    const Plan plans[] = {
+        // 8^3, FP32, 100 steps: rows of 64 padded cells (311,296 B states), 512 cells x 16 B moments
+        {100, 311296, 8192, 311296, 4 * 3 * 512 * 4, 50, 2 * 311296ull + 50 * 8192ull + 4 * 311296ull + 24576},
+        // 16^3, FP16 checkpoints of FP32 states, 400 steps
+        {400, 622592, 65536, 1245184, 4 * 3 * 4096 * 4, 67, 6 * 622592ull + 67 * 65536ull + 4 * 1245184ull + 196608},
+        // one step: one checkpoint, one slot
+        {1, 1000, 10, 1000, 0, 1, 1000 + 10 + 4000},
+    };
+    for (const Plan& q : plans) {
+        const int K = d3q19::adjoint_least_memory_K(q.T, q.ck, q.mom, q.st, q.fields);
+        const unsigned long long b = d3q19::adjoint_plan_bytes(q.T, K, q.ck, q.mom, q.st, q.fields);
+        bool least = true;
+        for (int k = 1; k <= q.T; ++k)
+            if (d3q19::adjoint_plan_bytes(q.T, k, q.ck, q.mom, q.st, q.fields) < b) least = false;
+        if (K != q.wantK || b != q.wantBytes || !least) {
+            std::printf("adjoint plan T %d: K %d (want %d), %llu B (want %llu), least %d\n", q.T, K, q.wantK, b,
+                        q.wantBytes, least);
+            ++bad;
+        }
+    }
+    // the benchmarked inverse problem: 128^3 FP32, 2000 steps, K = 50 -> 8.79 GB
+    {
+        const unsigned long long st = 128ull * 128 * 19 * 128 * 4, mom = 16ull * 128 * 128 * 128;
+        const unsigned long long b = d3q19::adjoint_plan_bytes(2000, 50, st, mom, st, 4 * 3 * 4 * 128ull * 128 * 128);
+        if (b != 8791261184ull) {
+            std::printf("adjoint plan 128^3: %llu B\n", b);
             ++bad;
         }
     }

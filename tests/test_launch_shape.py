@@ -1,8 +1,10 @@
 """Host-only test of the launch-shape arithmetic in
 bit-physics_amd/csrc/d3q19_launch.hpp (tests/cpp/test_launch.cpp, built with
 the host C++ compiler): the LDS reservation that caps the fused step's
-resident blocks per CU leaves room for exactly the intended number, and the
-plane chunks a large step is launched in cover the lattice once."""
+resident blocks per CU leaves room for exactly the intended number, the
+plane chunks a large step is launched in cover the lattice once, the ring gap
+of the moment-coded step, and the adjoint's memory plan picks the checkpoint
+interval that needs the least memory."""
 
 from __future__ import annotations
 
