@@ -522,19 +522,23 @@ int stage_out(const void* src, bool src_double, size_t n, void* dst, int dt, int
 namespace {
 template <class Tag, bool ROWS>
 void launch_moment_range_l(d3q19_lattice* L, const StepGrid& sg, const Params<typename Codec<Tag>::C>& p,
-                           const typename Codec<Tag>::raw* in) {
+                           const typename Codec<Tag>::raw* in, uint32_t per) {
     switch (L->cfg.boundary) {
         case D3Q19_BC_PERIODIC:
-            k_moment_range<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            k_moment_range<Tag, BC_PERIODIC, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials, per);
             break;
         case D3Q19_BC_YWALLS:
-            k_moment_range<Tag, BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            k_moment_range<Tag, BC_YWALLS, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials, per);
             break;
         default:
-            k_moment_range<Tag, BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials);
+            k_moment_range<Tag, BC_HALFWAY, ROWS><<<sg.grid, sg.block, 0, L->stream>>>(in, p, L->mr_partials, per);
             break;
     }
 }
+
+// Cells each thread of k_moment_range folds: up to 8, while the grid keeps
+// 2048 blocks or more.
+inline uint32_t moment_range_per(uint32_t nblocks) { return std::max(1u, std::min(8u, nblocks / 2048u)); }
 
 // stored: the moments of the stored state as it lies in the buffer (the
 // post-collision populations of the last step, or a loaded state) instead of
@@ -544,11 +548,17 @@ int launch_moment_range(d3q19_lattice* L, bool stored) {
     typedef typename Codec<Tag>::C C;
     Params<C> p = make_params<C>(L);
     if (stored) p.local = 1;
-    const StepGrid sg = step_grid<Tag>(L);
-    if (L->rows)
-        launch_moment_range_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur));
-    else
-        launch_moment_range_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur));
+    StepGrid sg = step_grid<Tag>(L);
+    const uint32_t per = moment_range_per(sg.nblocks);
+    if (L->rows) {
+        sg.grid.z = (sg.grid.z + per - 1) / per;
+        sg.nblocks = sg.grid.x * sg.grid.y * sg.grid.z;
+        launch_moment_range_l<Tag, true>(L, sg, p, buf_of<Tag>(L, L->cur), per);
+    } else {
+        sg.grid.x = (sg.grid.x + per - 1) / per;
+        sg.nblocks = sg.grid.x;
+        launch_moment_range_l<Tag, false>(L, sg, p, buf_of<Tag>(L, L->cur), per);
+    }
     HIPCHK(hipGetLastError());
     k_moment_range_final<<<2 * Q, 256, 0, L->stream>>>(L->mr_partials, sg.nblocks, L->mr_out);
     HIPCHK(hipGetLastError());

@@ -161,37 +161,52 @@ static __global__ void __launch_bounds__(256)
 
 // Per-block minimum and maximum of the 19 moments m = M f (FP64) of the
 // current state f^n, for calibrating the quantization ranges.  Partials:
-// [block][0..18] minima, [block][19..37] maxima.
+// [block][0..18] minima, [block][19..37] maxima.  Each thread folds `per`
+// cells (x planes blockIdx.z * per .. on the row grid, cell runs of the block
+// on the linear one) before the block's reduction: minima and maxima are
+// exact, so the grouping does not change the result, and the 38 wavefront
+// reductions (six FP64 shuffles each) are paid once per `per` cells.
 template <class Tag, int BC, bool ROWS>
 static __global__ void __launch_bounds__(step_block<Tag>())
     k_moment_range(const typename Codec<Tag>::raw* __restrict__ in, Params<typename Codec<Tag>::C> p,
-                   double* __restrict__ partials) {
+                   double* __restrict__ partials, uint32_t per) {
     typedef typename Codec<Tag>::C C;
     constexpr int NW = step_block<Tag>() / 64;
     __shared__ double red[NW][2 * Q];
     double lo[Q], hi[Q];
-    uint32_t x, y, z;
-    const bool live = cell_of<ROWS>(p.nx, p.ny, p.nz, p.ncell, x, y, z);
-    const uint32_t cell = live ? (x * p.ny + y) * p.nz + z : 0u;
-    if (live && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
-        C d[Q];
-        if (p.local)
-            load_local<Tag>(in, p, x, y, z, d);
-        else
-            pull<Tag, BC>(in, p, x, y, z, d);
 #pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int i = 0; i < Q; ++i)
-                if (me_mono(k, i) != 0) s += (double)me_mono(k, i) * (wgt<double>(i) + (double)d[i]);
-            lo[k] = hi[k] = s;
+    for (int k = 0; k < Q; ++k) {
+        lo[k] = 1e300;
+        hi[k] = -1e300;
+    }
+    for (uint32_t f = 0; f < per; ++f) {
+        uint32_t x, y, z;
+        bool live;
+        if constexpr (ROWS) {
+            live = grid_cell(p.nx, p.ny, p.nz, x, y, z, blockIdx.z * (per - 1u) + f);  // x = blockIdx.z per + f
+        } else {
+            const uint32_t c = (blockIdx.x * per + f) * blockDim.x + threadIdx.x;
+            live = c < p.ncell;
+            z = live ? c % p.nz : 0u;
+            y = live ? (c / p.nz) % p.ny : 0u;
+            x = live ? c / p.nz / p.ny : 0u;
         }
-    } else {
+        const uint32_t cell = live ? (x * p.ny + y) * p.nz + z : 0u;
+        if (live && !((BC == BC_HALFWAY) && p.solid && p.solid[cell])) {
+            C d[Q];
+            if (p.local)
+                load_local<Tag>(in, p, x, y, z, d);
+            else
+                pull<Tag, BC>(in, p, x, y, z, d);
 #pragma unroll
-        for (int k = 0; k < Q; ++k) {
-            lo[k] = 1e300;
-            hi[k] = -1e300;
+            for (int k = 0; k < Q; ++k) {
+                double s = 0.0;
+#pragma unroll
+                for (int i = 0; i < Q; ++i)
+                    if (me_mono(k, i) != 0) s += (double)me_mono(k, i) * (wgt<double>(i) + (double)d[i]);
+                lo[k] = fmin(lo[k], s);
+                hi[k] = fmax(hi[k], s);
+            }
         }
     }
     const int t = threadIdx.x + blockDim.x * threadIdx.y;
