@@ -353,6 +353,11 @@ __device__ __forceinline__ uint32_t block_linear_id() {
     return blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 }
 
+// Blocks of the grid.  Per-block partials of a reduction are stored
+// component-major, [component][block], so that the final reduction of a
+// component reads them contiguously.
+__device__ __forceinline__ uint32_t block_count() { return gridDim.x * gridDim.y * gridDim.z; }
+
 template <class C>
 __device__ __forceinline__ const char* row_of(const void* base, const Params<C>& p, uint32_t x, uint32_t y) {
     return (const char*)base + (uint64_t)(x * p.ny + y) * p.row_bytes;

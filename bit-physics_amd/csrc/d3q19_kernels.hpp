@@ -329,7 +329,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     if (t < NDIAG) {
         double s = 0.0;
         for (int w = 0; w < NW; ++w) s += red[w][t];
-        partials[(size_t)block_linear_id() * NDIAG + t] = s;
+        partials[(size_t)t * block_count() + block_linear_id()] = s;
     }
 }
 
@@ -387,7 +387,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
             v[2] = fmax(v[2], red[w][2]);
             v[3] = fmax(v[3], red[w][3]);
         }
-        for (int k = 0; k < NHEALTH; ++k) partials[(size_t)block_linear_id() * NHEALTH + k] = v[k];
+        for (int k = 0; k < NHEALTH; ++k) partials[(size_t)k * block_count() + block_linear_id()] = v[k];
     }
 }
 
@@ -398,7 +398,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     const int k = blockIdx.x;
     double v = k == 1 ? 1e300 : (k == 2 ? -1e300 : 0.0);
     for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
-        const double q = partials[(size_t)b * NHEALTH + k];
+        const double q = partials[(size_t)k * nblocks + b];
         v = k == 0 ? v + q : (k == 1 ? fmin(v, q) : fmax(v, q));
     }
 #pragma unroll
@@ -425,7 +425,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     const int k = blockIdx.x;
     if (k >= ncomp) return;
     double s = 0.0;
-    for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) s += partials[(size_t)b * ncomp + k];
+    for (uint32_t b = threadIdx.x; b < nblocks; b += BLOCK) s += partials[(size_t)k * nblocks + b];
     s = wave_sum(s);
     if (lane == 0) red[wave] = s;
     __syncthreads();

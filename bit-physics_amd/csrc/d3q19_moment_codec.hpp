@@ -161,7 +161,7 @@ static __global__ void __launch_bounds__(256)
 
 // Per-block minimum and maximum of the 19 moments m = M f (FP64) of the
 // current state f^n, for calibrating the quantization ranges.  Partials:
-// [block][0..18] minima, [block][19..37] maxima.  Each thread folds `per`
+// [0..18][block] minima, [19..37][block] maxima.  Each thread folds `per`
 // cells (x planes blockIdx.z * per .. on the row grid, cell runs of the block
 // on the linear one) before the block's reduction: minima and maxima are
 // exact, so the grouping does not change the result, and the 38 wavefront
@@ -229,7 +229,7 @@ static __global__ void __launch_bounds__(step_block<Tag>())
     if (t < 2 * Q) {
         double v = red[0][t];
         for (int w = 1; w < nw; ++w) v = t < Q ? fmin(v, red[w][t]) : fmax(v, red[w][t]);
-        partials[(size_t)block_linear_id() * 2 * Q + t] = v;
+        partials[(size_t)t * block_count() + block_linear_id()] = v;
     }
 }
 
@@ -242,7 +242,7 @@ static __global__ void __launch_bounds__(256)
     const bool is_min = t < Q;
     double v = is_min ? 1e300 : -1e300;
     for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
-        const double p = partials[(size_t)b * 2 * Q + t];
+        const double p = partials[(size_t)t * nblocks + b];
         v = is_min ? fmin(v, p) : fmax(v, p);
     }
 #pragma unroll
