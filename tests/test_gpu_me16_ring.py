@@ -33,13 +33,13 @@ def amd():
     return m
 
 
-def created(amd, shape, env, **kw):
+def created(amd, shape, env, tau=TAU, **kw):
     saved = {k: os.environ.get(k) for k in ("D3Q19_RING_GAP", "D3Q19_STEP_CHUNKS", "D3Q19_RING_FAULT_TEST")}
     try:
         for k in saved:
             os.environ.pop(k, None)
         os.environ.update(env)
-        return amd.Lattice(shape, TAU, storage="me16", **kw)
+        return amd.Lattice(shape, tau, storage="me16", **kw)
     finally:
         for k, v in saved.items():
             if v is None:
@@ -129,6 +129,28 @@ def test_ring_on_ragged_lattices(amd, name, bc):
         assert np.array_equal(runs[run][0], ref[0]), run
         assert np.array_equal(runs[run][1], ref[1]), run
         assert runs[run][2] == ref[2], run
+
+
+def test_ring_at_the_benchmarked_size(amd):
+    """256^3 Taylor-Green, the moment-coded leg's workload: the step in one
+    launch with the automatic gap (18 planes, every CU's blocks resident)
+    equals the plane-by-plane step bit for bit after 10 steps."""
+    from lattice_boltzmann_d3q19_amd.initial import taylor_green
+    from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+    shape, tau, steps = (256, 256, 256), 0.55, 10
+    rho0, u0 = taylor_green(shape, 0.04)
+    lo, hi = calibrate_moment_ranges(shape, tau, steps, rho=rho0, u=u0)
+    states = {}
+    for run, env in (("planes", {"D3Q19_STEP_CHUNKS": "256"}), ("auto", {})):
+        print_case("me16-ring-256", run=run, steps=steps)
+        lat = created(amd, shape, env, tau=tau)
+        lat.set_moment_ranges(lo, hi).set_equilibrium(rho0, u0).step(steps)
+        assert lat.step_launches == (256 if run == "planes" else 1)
+        states[run] = (lat.raw_state()[0], lat.diagnostics())
+        lat.close()
+    assert np.array_equal(states["auto"][0], states["planes"][0])
+    assert states["auto"][1] == states["planes"][1]
 
 
 def test_ring_holds_about_half_the_memory_of_fp16(amd):
