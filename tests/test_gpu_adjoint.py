@@ -42,11 +42,12 @@ def gpu_grad(torch, ag, shape, tau, T, K, storage, u0, F, ut):
     return float(loss.detach()), u0t.grad.cpu().numpy(), Ft.grad.cpu().numpy(), solver
 
 
-@pytest.mark.parametrize("T,K", [(7, 3), (6, 6), (5, 1)])
-def test_fp64_adjoint_matches_oracle_adjoint(mods, T, K):
+# (5, 7, 70): rows of 70 cells in blocks of 128 z x 2 rows (58 padding lanes, a last block
+# with one live row), as the recomputation's moments and the reverse step see them
+@pytest.mark.parametrize("T,K,shape", [(7, 3, (6, 5, 4)), (6, 6, (6, 5, 4)), (5, 1, (6, 5, 4)), (7, 3, (5, 7, 70))])
+def test_fp64_adjoint_matches_oracle_adjoint(mods, T, K, shape):
     torch, ag = mods
-    print_case("adjoint-fp64-vs-oracle", T=T, K=K)
-    shape = (6, 5, 4)
+    print_case("adjoint-fp64-vs-oracle", T=T, K=K, shape=shape)
     u0, F, ut = problem(shape)
     L, gu, gF, _ = gpu_grad(torch, ag, shape, 0.6, T, K, "f64", u0, F, ut)
     Lr, gur, gFr = oracle.inverse_grad(u0, F, ut, 0.6, T)
