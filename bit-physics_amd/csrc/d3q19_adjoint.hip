@@ -148,12 +148,13 @@ __global__ void __launch_bounds__(step_block<Tag>())
 #pragma unroll
     for (int i = 0; i < Q; ++i) {
         // one load per direction from a selected address (no load inside a
-        // branch arm, whose join would wait for every load in flight)
+        // branch arm, whose join would wait for every load in flight); the
+        // nontemporal hint on the c_z = 0 directions only, as the step's pull
         const bool back = BC == BC_HALFWAY && crosses_wall(-CX[i], -CY[i], -CZ[i], x, y, z, p);
         const char* src = back ? self_row + (uint32_t)OPP[i] * p.dir_bytes + z * ES
                                : row_of(lam_in, p, xs[1 + CX[i]], ys[1 + CY[i]]) + (uint32_t)i * p.dir_bytes +
                                      zs[1 + CZ[i]] * ES;
-        mu[i] = ld_at<Tag, true>(src, i);
+        mu[i] = CZ[i] == 0 ? ld_at<Tag, true>(src, i) : ld_at<Tag, false>(src, i);
     }
     const Vec4<C> m4 = __builtin_nontemporal_load((const Vec4<C>*)mom + cell);
     const C drho = m4.x, mx = m4.y, my = m4.z, mz = m4.w;
