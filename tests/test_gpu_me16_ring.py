@@ -131,6 +131,52 @@ def test_ring_on_ragged_lattices(amd, name, bc):
         assert runs[run][2] == ref[2], run
 
 
+def _sweep_cases(n=8, seed=20261204):
+    """Seeded row-grid lattices: nx 3-40, ny 3-40, nz 64-300 (padded rows, 64- to
+    256-cell z blocks), a boundary mode, a gap of 2 up to the automatic one,
+    and 1-4 plane chunks."""
+    rng = np.random.default_rng(seed)
+    bcs = ("periodic", "forced", "forced-ywalls", "cavity")
+    out = []
+    for k in range(n):
+        shape = (int(rng.integers(3, 41)), int(rng.integers(3, 41)), int(rng.integers(64, 301)))
+        out.append((shape, bcs[k % len(bcs)], int(rng.integers(2, 9)), int(rng.integers(1, 5))))
+    return out
+
+
+SWEEP_KW = {
+    "periodic": dict(),
+    "forced": dict(force=(2e-6, 1e-6, -3e-6)),
+    "forced-ywalls": dict(boundary="ywalls", force=(1e-5, 0.0, 0.0), wall_velocity_top=(0.02, 0.0, 0.01)),
+    "cavity": dict(boundary="halfway", wall_faces=("-x", "+x", "-y", "+y", "-z", "+z"),
+                   face_velocity={"+z": (0.05, 0.0, 0.0)}),
+}
+
+
+@pytest.mark.parametrize("shape,bc,gap,chunks", _sweep_cases())
+def test_ring_on_seeded_lattices(amd, shape, bc, gap, chunks):
+    """The ring step, at the automatic gap and at a seeded tighter gap with
+    seeded plane chunks, equals the plane-by-plane step bit for bit."""
+    from lattice_boltzmann_d3q19_amd.initial import random_solenoidal
+    from lattice_boltzmann_d3q19_amd.moment_encoded import calibrate_moment_ranges
+
+    kw = SWEEP_KW[bc]
+    u0 = random_solenoidal(shape, 0.02, seed=shape[0] * 7 + shape[2])
+    steps = 7
+    lo, hi = calibrate_moment_ranges(shape, TAU, steps, u=u0, **kw)
+    runs = {}
+    for run, env in (("planes", {"D3Q19_STEP_CHUNKS": str(shape[0])}), ("auto", {}),
+                     ("seeded", {"D3Q19_RING_GAP": str(gap), "D3Q19_STEP_CHUNKS": str(chunks)})):
+        print_case("me16-ring-sweep", shape=shape, bc=bc, run=run, gap=gap, chunks=chunks)
+        lat = created(amd, shape, env, **kw)
+        lat.set_moment_ranges(lo, hi).set_equilibrium(None, u0).step(steps)
+        runs[run] = (lat.raw_state()[0].copy(), lat.diagnostics())
+        lat.close()
+    for run in ("auto", "seeded"):
+        assert np.array_equal(runs[run][0], runs["planes"][0]), run
+        assert runs[run][1] == runs["planes"][1], run
+
+
 def test_ring_at_the_benchmarked_size(amd):
     """256^3 Taylor-Green, the moment-coded leg's workload: the step in one
     launch with the automatic gap (18 planes, every CU's blocks resident)
