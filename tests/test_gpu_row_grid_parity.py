@@ -1,0 +1,105 @@
+"""GPU parity of the fused step on the row grid against the FP64 oracle, on
+seeded lattices whose rows are not a whole number of blocks (8 cases).
+
+Lattices with rows of 64 cells or more run on the row grid: rows padded to a
+multiple of 64, blocks of 64-256 z cells times several rows, padding lanes
+and partly live row blocks.  Each case draws a shape (nz 65-300), a boundary
+mode and its parameters from a seeded generator and compares the populations
+after 30 steps with the oracle's: FP64 storage within 1e-12 and FP32 storage
+within 1e-6, field-scaled (the tolerances of the fixed-shape parity tests).
+
+* periodic with a per-cell Guo force field (NP/reference/bgk.py:45-216 loop,
+  oracle.run_reference);
+* the reference's y-walls with moving plates and a force field
+  (NP/sim.py:312-326, oracle.run_reference with ywalls);
+* half-way walls on a seeded subset of faces (z faces included) with a moving
+  face, 10% solid cells, a Brinkman field and a uniform force
+  (oracle.run_halfway), compared on fluid cells.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import field_scaled_err, print_case
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-12, "f32": 1e-6}
+STEPS = 30
+FACES = ("-x", "+x", "-y", "+y", "-z", "+z")  # wall_mask bits 1, 2, 4, 8, 16, 32
+
+
+@pytest.fixture(scope="module")
+def amd():
+    import lattice_boltzmann_d3q19_amd as m
+
+    return m
+
+
+def _cases(n=8, seed=7041):
+    rng = np.random.default_rng(seed)
+    modes = ("periodic", "ywalls", "halfway")
+    out = []
+    for k in range(n):
+        shape = (int(rng.integers(2, 7)), int(rng.integers(2, 9)),
+                 int(rng.choice([65, 70, 100, 129, 200, 257, 300])))
+        out.append((shape, modes[k % len(modes)], int(rng.integers(0, 1 << 30))))
+    return out
+
+
+def _seeded_state(shape, rng):
+    rho = 1.0 + 0.01 * rng.standard_normal(shape)
+    u = 0.02 * rng.standard_normal((3,) + shape)
+    f = oracle.feq_field(rho, u) * (1.0 + 0.02 * rng.standard_normal((19,) + shape))
+    F = 1e-4 * rng.standard_normal((3,) + shape)
+    return f, F
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("shape,mode,seed", _cases(), ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
+def test_row_grid_step_matches_oracle(amd, storage, shape, mode, seed):
+    print_case("row-grid-parity", shape=shape, mode=mode, seed=seed, storage=storage)
+    rng = np.random.default_rng(seed)
+    f0, F = _seeded_state(shape, rng)
+    tau = float(rng.uniform(0.55, 0.9))
+    fluid = np.ones(shape, bool)
+    if mode == "periodic":
+        lat = amd.Lattice(shape, tau, storage=storage, force=F)
+        lat.set_populations(f0).step(STEPS)
+        ref = oracle.run_reference(f0, tau, STEPS, force_lattice=F)
+    elif mode == "ywalls":
+        top = tuple(0.03 * rng.standard_normal(3))
+        bot = tuple(0.03 * rng.standard_normal(3))
+        lat = amd.Lattice(shape, tau, storage=storage, boundary="ywalls", wall_velocity_top=top,
+                          wall_velocity_bottom=bot, force=F)
+        lat.set_populations(f0).step(STEPS)
+        ref = oracle.run_reference(f0, tau, STEPS, force_lattice=F, ywalls=True,
+                                   wall_velocity_top=top, wall_velocity_bottom=bot)
+    else:
+        mask = int(rng.integers(1, 64)) | 48  # the z faces always, others seeded
+        faces = tuple(name for b, name in enumerate(FACES) if mask >> b & 1)
+        moving = faces[int(rng.integers(0, len(faces)))]
+        uface = tuple(0.04 * rng.standard_normal(3))
+        fv = np.zeros((6, 3))
+        fv[FACES.index(moving)] = uface
+        solid = rng.random(shape) < 0.1
+        brink = np.where(rng.random(shape) < 0.3, 0.5 * rng.random(shape), 0.0)
+        fu = (1e-5, -2e-6, 3e-6)
+        rho_eq = np.ones(shape)
+        u_eq = 0.01 * rng.standard_normal((3,) + shape)
+        f0 = oracle.feq_field(rho_eq, u_eq)
+        lat = amd.Lattice(shape, tau, storage=storage, boundary="halfway", wall_faces=faces,
+                          face_velocity={moving: uface}, force=fu)
+        lat.set_solid(solid).set_brinkman(brink).set_populations(f0).step(STEPS)
+        ref = oracle.run_halfway(f0, tau, STEPS, wall_mask=mask, face_velocity=fv, solid=solid,
+                                 force_uniform=fu, brinkman=brink)
+        fluid = ~solid
+    got = lat.populations()
+    lat.close()
+    assert np.isfinite(got).all()
+    err = field_scaled_err(got[:, fluid], ref[:, fluid])
+    print(f"field-scaled error {err:.2e}", flush=True)
+    assert err < TOL[storage]
