@@ -78,8 +78,8 @@ def _seeded_state(shape, rng):
 
 @pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16sr"])
 @pytest.mark.parametrize("shape,mode,seed", _cases() + _short_cases(), ids=lambda v: "x".join(map(str, v)) if isinstance(v, tuple) else str(v))
-def test_row_grid_step_matches_oracle(amd, storage, shape, mode, seed):
-    print_case("row-grid-parity", shape=shape, mode=mode, seed=seed, storage=storage)
+def test_seeded_step_matches_oracle(amd, storage, shape, mode, seed):
+    print_case("seeded-parity", shape=shape, mode=mode, seed=seed, storage=storage)
     rng = np.random.default_rng(seed)
     f0, F = _seeded_state(shape, rng)
     tau = float(rng.uniform(0.55, 0.9))
